@@ -1,0 +1,163 @@
+/* pharos_synth.h -- C ABI of the MI355X IF sample engine (gfx950 HIP kernels).
+ *
+ * This is the seam the reference names for "another kernel, such as a GPU" (docs/design/architecture.md:341):
+ * it replaces the call of kernel.synthSubTile from Run.runTask (src/products/run/engine/run.zig:277-296) and the
+ * worker fan-out of Run.runJob (run.zig:367-380).  The host keeps the epoch-rate stage (geometry, channel plans,
+ * EpochParams derivation, symbol masks) and hands this library, per window of epochs, one row per channel and
+ * epoch plus the mask bytes those rows reference.  Every pointer is plain memory; no framework type crosses.
+ *
+ * Memory: pointers documented as "device" are HIP device memory (hipMalloc, or any allocator of the same device);
+ * the library reads and writes them on the given stream (NULL = the default stream).  Host pointers are ordinary
+ * process memory.
+ *
+ * Status codes are those of pharos_run.h (src/interface/c/status.zig:5): 0 ok, 2 argument, 3 out of memory,
+ * 4 internal.  After a non-zero status, pharos_synth_last_error() holds a message (thread-local).
+ */
+#ifndef PHAROS_SYNTH_H
+#define PHAROS_SYNTH_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHAROS_SYNTH_ABI_VERSION 0x00010000u
+#define PHAROS_SYNTH_INACTIVE 0xFFFFFFFFu
+
+/* Sample formats: IqFormat of src/simulation/model/sampling.zig:31-35. */
+enum { PHAROS_SYNTH_IQ2 = 0, PHAROS_SYNTH_IQ4 = 1, PHAROS_SYNTH_IQ8 = 2, PHAROS_SYNTH_IQ16 = 3 };
+/* Diagnostic format with no counterpart in the reference: each sample's pre-quantisation sum (after noise) as two
+ * little-endian f32, I then Q; the gain is ignored and nothing clips. */
+enum { PHAROS_SYNTH_F32 = 4 };
+/* Noise modes: noise.Mode of src/products/run/synthesis/noise.zig:8-13. */
+enum { PHAROS_SYNTH_NOISE_COUNTER = 0, PHAROS_SYNTH_NOISE_OFF = 1 };
+/* Gain modes: gain.Mode of src/products/run/synthesis/gain.zig:45. */
+enum { PHAROS_SYNTH_GAIN_ADAPTIVE = 0, PHAROS_SYNTH_GAIN_CONSTANT = 1 };
+
+/* kernel.Config (src/products/run/synthesis/kernel.zig:31-37), less the sub-tile, which the engine chooses. */
+typedef struct pharos_synth_config {
+    uint32_t samples_per_epoch; /* sample rate in kHz; 512 .. 200000 */
+    uint32_t format;            /* PHAROS_SYNTH_IQ* or PHAROS_SYNTH_F32; IQ2 needs an even samples_per_epoch */
+    uint32_t noise;             /* PHAROS_SYNTH_NOISE_* */
+    uint32_t reserved;          /* 0 */
+    uint64_t seed;
+} pharos_synth_config;
+
+/* One code table, as code_tables.Table (src/products/run/synthesis/code_tables.zig:9-29) holds it.  Host memory;
+ * copied at create. */
+typedef struct pharos_synth_table {
+    const uint8_t *states;       /* entries_per_period folded states, each < value_count */
+    uint32_t entries_per_period;
+    uint32_t entries_per_ms;
+    uint32_t value_count;        /* 4 or 8 */
+    uint32_t tdm_periods;        /* pilot period / code period for a TDM table (L2C), else 0 */
+    const uint8_t *tdm_pilot;    /* TDM pilot chips (0/1), tdm_pilot_len of them; NULL unless TDM */
+    uint32_t tdm_pilot_len;
+    uint32_t reserved;
+    double values_re[8];         /* Spec.stateValue per state (src/domain/signals/model.zig:151-166) */
+    double values_im[8];
+} pharos_synth_table;
+
+/* parameters.EpochParams (src/products/run/synthesis/parameters.zig:13-30), one per channel and epoch.  The
+ * reference's `masks` slice becomes an offset into one mask buffer: masks[mask_offset + k] is the mask byte of
+ * code period period0 + k, k < mask_count.  64 bytes. */
+typedef struct pharos_synth_row {
+    double carrier_phase0;       /* cycles in [0, 1) */
+    double carrier_step;         /* cycles per sample */
+    double entry_phase0;         /* code entries from the start of period0 */
+    double entries_per_sample;
+    double lead_entries;         /* entries the leading component is ahead; 0 without a lead */
+    uint64_t period0;
+    uint32_t table;              /* index into the engine's tables; PHAROS_SYNTH_INACTIVE for a null row */
+    float amp;
+    uint32_t mask_offset;
+    uint16_t mask_count;
+    uint8_t lead_bits;           /* state bits of the leading component; 0 without a lead */
+    uint8_t flags;               /* 0 */
+} pharos_synth_row;
+
+typedef struct pharos_synth_engine pharos_synth_engine;
+
+/* Automatic gain state of one run (gain.zig:14-35): the linear gain and the counts since the last evaluation. */
+typedef struct pharos_synth_agc {
+    double gain;
+    uint64_t clipped;
+    uint64_t components;
+    uint32_t adjustments;        /* number of gain changes recorded so far (output) */
+    uint32_t reserved;
+} pharos_synth_agc;
+
+/* One AGC evaluation that changed the gain (gain.Adjustment, gain.zig:12). */
+typedef struct pharos_synth_adjustment {
+    uint64_t epoch;
+    double gain;
+    double clip_rate;
+    uint32_t reduced;
+    uint32_t reserved;
+} pharos_synth_adjustment;
+
+uint32_t pharos_synth_abi_version(void);
+const char *pharos_synth_last_error(void);
+
+/* Creates an engine on HIP device `device` (the tables are copied to it).  Replaces the table part of
+ * Channels.create (src/products/run/engine/channels.zig:127-203) on the device side. */
+int pharos_synth_create(const pharos_synth_config *config, const pharos_synth_table *tables, uint32_t table_count,
+                        int device, pharos_synth_engine **out);
+void pharos_synth_destroy(pharos_synth_engine *engine);
+uint64_t pharos_synth_bytes_per_epoch(const pharos_synth_engine *engine);
+/* Device bytes the engine holds (tables), for admission by free memory. */
+uint64_t pharos_synth_device_bytes(const pharos_synth_engine *engine);
+
+/* Plain form: one launch synthesises `epochs` epochs from first_epoch under one gain, like one window of
+ * Run.runJob(.produce) (run.zig:367-380).  rows (device) is [epochs][channels]; masks (device) holds mask_len
+ * bytes; out (device) receives epochs * bytes_per_epoch bytes.  clipped (host, optional) receives one count per
+ * epoch.  Synchronises the stream when clipped is given. */
+int pharos_synth_window(pharos_synth_engine *engine, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+                        const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
+                        uint8_t *out, uint64_t *clipped, void *stream);
+
+/* Batched form: many runs (jobs) of any engines, rates and formats in one persistent launch over a work queue of
+ * (job, window, tile) items.  Windows are the AGC evaluation windows (epoch multiples of 100, gain.zig:5) and the
+ * adaptive gain is carried on the device from window to window, so no host round trip separates windows. */
+typedef struct pharos_synth_job {
+    pharos_synth_engine *engine;
+    uint64_t first_epoch;
+    uint32_t epochs;
+    uint32_t channels;
+    const pharos_synth_row *rows;     /* device, [epochs][channels] */
+    const uint8_t *masks;             /* device */
+    uint64_t mask_len;
+    uint8_t *out;                     /* device, epochs * bytes_per_epoch */
+    uint32_t gain_mode;               /* PHAROS_SYNTH_GAIN_* */
+    float constant_gain;              /* used when gain_mode is constant */
+    pharos_synth_agc *agc;            /* host; in: state before first_epoch; out: state after (adaptive mode) */
+    pharos_synth_adjustment *adjustments; /* host, optional: receives the gain changes in order */
+    uint32_t adjustments_cap;
+    uint32_t reserved;
+    uint64_t *clipped;                /* host, optional: total clipped components of the job */
+} pharos_synth_job;
+
+/* Launch shape; zero fields take the tuned defaults. */
+typedef struct pharos_synth_tuning {
+    uint32_t workgroups_per_cu;      /* persistent workgroups per compute unit */
+    uint32_t reserved[3];
+} pharos_synth_tuning;
+
+/* Runs all jobs to completion on `stream` and synchronises it.  Each job's bytes equal what the plain form
+ * produces for it alone. */
+int pharos_synth_batch(const pharos_synth_job *jobs, uint32_t job_count, const pharos_synth_tuning *tuning,
+                       void *stream);
+
+/* Asynchronous batched form for serving loops: enqueues the batch and returns; pharos_synth_batch_wait finishes it
+ * (copies AGC state and counts back).  One pending batch per handle. */
+typedef struct pharos_synth_batch_handle pharos_synth_batch_handle;
+int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t job_count, const pharos_synth_tuning *tuning,
+                              void *stream, pharos_synth_batch_handle **out);
+int pharos_synth_batch_query(pharos_synth_batch_handle *handle); /* 1 done, 0 running, <0 error */
+int pharos_synth_batch_wait(pharos_synth_batch_handle *handle);  /* frees the handle */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,143 @@
+"""ctypes binding of libpharos_amd.so (include/pharos_synth.h, include/pharos_run.h, csrc/pharos_host.h).
+
+The library is loaded from this package directory only; importing a function raises when it is missing, so no
+caller silently falls back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpharos_amd.so")
+DATA_DIR = os.path.join(HERE, "data")
+
+# ----------------------------------------------------------------------------- records
+class SynthConfig(C.Structure):
+    _fields_ = [("samples_per_epoch", C.c_uint32), ("format", C.c_uint32), ("noise", C.c_uint32),
+                ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class SynthTable(C.Structure):
+    _fields_ = [("states", C.POINTER(C.c_uint8)), ("entries_per_period", C.c_uint32), ("entries_per_ms", C.c_uint32),
+                ("value_count", C.c_uint32), ("tdm_periods", C.c_uint32), ("tdm_pilot", C.POINTER(C.c_uint8)),
+                ("tdm_pilot_len", C.c_uint32), ("reserved", C.c_uint32), ("values_re", C.c_double * 8),
+                ("values_im", C.c_double * 8)]
+
+
+class SynthAgc(C.Structure):
+    _fields_ = [("gain", C.c_double), ("clipped", C.c_uint64), ("components", C.c_uint64),
+                ("adjustments", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class SynthAdjustment(C.Structure):
+    _fields_ = [("epoch", C.c_uint64), ("gain", C.c_double), ("clip_rate", C.c_double), ("reduced", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class SynthJob(C.Structure):
+    _fields_ = [("engine", C.c_void_p), ("first_epoch", C.c_uint64), ("epochs", C.c_uint32), ("channels", C.c_uint32),
+                ("rows", C.c_void_p), ("masks", C.c_void_p), ("mask_len", C.c_uint64), ("out", C.c_void_p),
+                ("gain_mode", C.c_uint32), ("constant_gain", C.c_float), ("agc", C.POINTER(SynthAgc)),
+                ("adjustments", C.POINTER(SynthAdjustment)), ("adjustments_cap", C.c_uint32),
+                ("reserved", C.c_uint32), ("clipped", C.POINTER(C.c_uint64))]
+
+
+class SynthTuning(C.Structure):
+    _fields_ = [("workgroups_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class SignalConstants(C.Structure):
+    _fields_ = [("system", C.c_uint32), ("code_ms", C.c_uint32), ("entries_per_chip", C.c_uint32),
+                ("entries_per_ms", C.c_uint32), ("entries_per_period", C.c_uint32), ("value_count", C.c_uint32),
+                ("tdm_periods", C.c_uint32), ("state_bits_pilot", C.c_uint32), ("state_bits_data", C.c_uint32),
+                ("nh_length", C.c_uint32), ("nh_code", C.c_uint32), ("frame_length_ms", C.c_uint32),
+                ("frame_offset_ms", C.c_uint32), ("geo_nh_length", C.c_uint32), ("geo_nh_code", C.c_uint32),
+                ("geo_frame_length_ms", C.c_uint32), ("secondary_length", C.c_uint32), ("satellites", C.c_uint32),
+                ("carrier_hz", C.c_int64), ("fdma_step_hz", C.c_int64)]
+
+
+class ChannelSpec(C.Structure):
+    _fields_ = [("signal", C.c_uint32), ("prn", C.c_uint32), ("freq_k", C.c_int32), ("lead_component", C.c_uint32),
+                ("if_hz", C.c_int64), ("range0_m", C.c_double), ("doppler_hz", C.c_double),
+                ("doppler_rate_hz_s", C.c_double), ("wobble_accel_mps2", C.c_double), ("wobble_period_s", C.c_double),
+                ("cn0_dbhz", C.c_double), ("lead_entries", C.c_double), ("nav_seed", C.c_uint64)]
+
+
+class ScenarioSpec(C.Structure):
+    _fields_ = [("samples_per_epoch", C.c_uint32), ("channel_count", C.c_uint32), ("start_gps_ms", C.c_int64),
+                ("leap_seconds", C.c_int32), ("reserved", C.c_uint32)]
+
+
+# pharos_synth_row as a numpy record (64 bytes, include/pharos_synth.h)
+ROW_DTYPE = np.dtype([("carrier_phase0", "<f8"), ("carrier_step", "<f8"), ("entry_phase0", "<f8"),
+                      ("entries_per_sample", "<f8"), ("lead_entries", "<f8"), ("period0", "<u8"), ("table", "<u4"),
+                      ("amp", "<f4"), ("mask_offset", "<u4"), ("mask_count", "<u2"), ("lead_bits", "u1"),
+                      ("flags", "u1")])
+assert ROW_DTYPE.itemsize == 64
+INACTIVE = 0xFFFFFFFF
+
+SIGNALS = ["L1CA", "L1C", "L2C", "L5", "B1C", "B1I", "B2I", "B3I", "B2a", "B2b", "E1", "E5a", "E5b", "E6", "G1", "G2"]
+SIGNAL = {name: i for i, name in enumerate(SIGNALS)}
+FORMATS = {"IQ2": 0, "IQ4": 1, "IQ8": 2, "IQ16": 3}
+BYTES_PER_SAMPLE = {0: 0.5, 1: 1, 2: 2, 3: 4}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises FileNotFoundError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: run `python -m pharos_amd.build` (HIP engine not built)")
+    L = C.CDLL(LIB_PATH)
+    P, VP, U32, U64, I32, I64, F32 = C.POINTER, C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_int64, C.c_float
+    sig = {
+        "pharos_synth_abi_version": (U32, []),
+        "pharos_synth_last_error": (C.c_char_p, []),
+        "pharos_synth_create": (C.c_int, [P(SynthConfig), P(SynthTable), U32, C.c_int, P(VP)]),
+        "pharos_synth_destroy": (None, [VP]),
+        "pharos_synth_bytes_per_epoch": (U64, [VP]),
+        "pharos_synth_device_bytes": (U64, [VP]),
+        "pharos_synth_window": (C.c_int, [VP, U64, U32, U32, VP, VP, U64, F32, VP, P(U64), VP]),
+        "pharos_synth_batch": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP]),
+        "pharos_synth_batch_submit": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP, P(VP)]),
+        "pharos_synth_batch_query": (C.c_int, [VP]),
+        "pharos_synth_batch_wait": (C.c_int, [VP]),
+        "pharos_host_set_data_dir": (C.c_int, [C.c_char_p]),
+        "pharos_signal_info": (C.c_int, [U32, P(SignalConstants)]),
+        "pharos_primary_code": (C.c_int, [U32, U32, C.c_int, P(C.c_uint8), U32]),
+        "pharos_secondary_code": (C.c_int, [U32, U32, P(C.c_uint8), U32]),
+        "pharos_table_build": (C.c_int, [U32, U32, P(SynthTable)]),
+        "pharos_table_release": (None, [P(SynthTable)]),
+        "pharos_scenario_window": (I64, [P(ScenarioSpec), P(ChannelSpec), U64, U32, VP, VP, U64]),
+        "pharos_scenario_mask_bound": (U64, [P(ScenarioSpec), P(ChannelSpec), U32]),
+        "pharos_scenario_endpoint": (None, [P(ScenarioSpec), P(ChannelSpec), I64, P(C.c_double), P(I64), P(C.c_double)]),
+        "pharos_derive": (C.c_int, [P(ScenarioSpec), P(ChannelSpec), U32, U64, VP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    L.pharos_host_set_data_dir(DATA_DIR.encode())
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().pharos_synth_last_error().decode(errors="replace")
+
+
+class PharosError(RuntimeError):
+    """A non-zero status from the library, with its message."""
+
+    def __init__(self, status, message):
+        super().__init__(f"status {status}: {message}")
+        self.status = status
+
+
+def check(status):
+    if status != 0:
+        raise PharosError(status, last_error())

@@ -1,0 +1,64 @@
+"""Builds libpharos_amd.so in-tree: gfx950 HIP kernels (hipcc) plus the host C++ (g++).
+
+Run as ``python -m pharos_amd.build``.  Every HIP object is built with -ffp-contract=off (the reference fuses
+only where it writes @mulAdd) and without packed FP32 instructions.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpharos_amd.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = os.environ.get("PHAROS_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["synth_engine.hip"]
+CXX_SOURCES = ["signals.cpp", "scenario.cpp", "run_facade.cpp"]
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-parameter", "-Wno-unused-function",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-parameter",
+             f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    """Compiles every source and links libpharos_amd.so next to this file (replaced atomically)."""
+    if not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    tmp = tempfile.mkdtemp(prefix="pharos_build_")
+    try:
+        objs = []
+        for src in HIP_SOURCES:
+            obj = os.path.join(tmp, src + ".o")
+            _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
+        for src in CXX_SOURCES:
+            path = os.path.join(CSRC, src)
+            if not os.path.exists(path):
+                continue
+            obj = os.path.join(tmp, src + ".o")
+            _run(["g++", *CXX_FLAGS, "-c", path, "-o", obj])
+            objs.append(obj)
+        out_tmp = LIB + ".tmp"
+        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", out_tmp, *objs, "-lpthread"])
+        os.replace(out_tmp, LIB)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,194 @@
+// Synthetic epoch-rate stage: endpoints from a seeded range model, rows by parameters.derive, symbol masks by
+// SymbolSource.fill.  It stands where Channels.prepare/deriveOne (src/products/run/engine/channels.zig:267-358)
+// stands in the reference, for scenarios described directly by their channels instead of a .phb bundle.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "pharos_host.h"
+
+namespace {
+
+constexpr double LIGHT = 299792458.0;
+constexpr int64_t BDS_OFFSET_MS = 14000;        // domain/time/gps.zig:20
+constexpr int64_t MOSCOW_OFFSET_MS = 3 * 3600000; // UTC+3 (domain/time/civil.zig)
+
+int64_t div_floor(int64_t a, int64_t b) { int64_t q = a / b; return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q; }
+int64_t mod_floor(int64_t a, int64_t b) { int64_t r = a % b; return r < 0 ? r + b : r; }
+
+double carrier_hz(const pharos_signal_constants &k, const pharos_channel_spec &c) {
+    return (double)(k.carrier_hz + k.fdma_step_hz * (int64_t)c.freq_k);
+}
+
+// GpsTime.baseMs (src/domain/time/gps.zig:109-115).
+int64_t base_ms(uint32_t system, int64_t ms, int32_t leap) {
+    switch (system) {
+    case 1: return ms - BDS_OFFSET_MS;
+    case 3: return ms - (int64_t)leap * 1000 + MOSCOW_OFFSET_MS;
+    default: return ms;
+    }
+}
+
+// parameters.ifPhase (parameters.zig:72-76): exact on whole milliseconds.
+double if_phase(int64_t if_hz, int64_t t_ms, double t_sub_ms) {
+    const int64_t num = mod_floor(mod_floor(if_hz, 1000) * mod_floor(t_ms, 1000), 1000);
+    const double whole = (double)num / 1000.0;
+    const double v = whole + (double)if_hz * t_sub_ms / 1000.0;
+    return v - floor(v);
+}
+
+uint64_t mix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    return x ^ (x >> 33);
+}
+
+struct Timing {
+    uint32_t code_ms, nh_len, nh_code, frame_ms, offset_ms;
+};
+
+// catalog.frameFor (src/domain/signals/catalog.zig:69-71): BDS GEO satellites (PRN <= 5 or >= 59) use the GEO frame.
+Timing timing_of(const pharos_signal_constants &k, const pharos_channel_spec &c) {
+    Timing t{k.code_ms, k.nh_length, k.nh_code, k.frame_length_ms, k.frame_offset_ms};
+    if (k.system == 1 && k.geo_frame_length_ms && (c.prn <= 5 || c.prn >= 59)) {
+        t.nh_len = k.geo_nh_length;
+        t.nh_code = k.geo_nh_code;
+        t.frame_ms = k.geo_frame_length_ms;
+    }
+    return t;
+}
+
+// SymbolSource.fill (src/products/run/symbols/source.zig:53-89), with a random nav bit per (frame, bit).
+void fill_masks(const pharos_signal_constants &k, const pharos_channel_spec &c, uint64_t k0, uint64_t n, const uint8_t *secondary,
+                uint32_t sec_len, uint8_t *out) {
+    const Timing t = timing_of(k, c);
+    const uint64_t bit_ms = (uint64_t)t.code_ms * t.nh_len;
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t kk = k0 + i;
+        const uint64_t ms = kk * t.code_ms + t.offset_ms;
+        const uint64_t in_frame = ms % t.frame_ms;
+        const uint32_t nh_pos = (uint32_t)((in_frame % bit_ms) / t.code_ms);
+        const uint64_t frame = ms / t.frame_ms, bit = in_frame / bit_ms;
+        const uint8_t nav = (uint8_t)(mix64(c.nav_seed ^ mix64(frame * 0x10001ull + bit)) & 1u);
+        const uint8_t nh = (uint8_t)((t.nh_code >> nh_pos) & 1u);
+        uint8_t m = (uint8_t)(nav ^ nh);
+        if (sec_len) m |= (uint8_t)(secondary[kk % sec_len] << 1);
+        out[i] = m;
+    }
+}
+
+// parameters.masksNeeded (parameters.zig:25-29).
+uint64_t masks_needed(const pharos_synth_row &p, uint32_t spe, uint32_t per) {
+    const uint32_t strided = (spe + 15) / 16 * 16;
+    const double end = p.entry_phase0 + (double)strided * p.entries_per_sample + p.lead_entries;
+    return (uint64_t)floor(end / (double)per) + 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pharos_scenario_endpoint(const pharos_scenario_spec *s, const pharos_channel_spec *c, int64_t t_rx_ms, double *phase_cycles,
+                              int64_t *tx_ms, double *tx_sub_ms) {
+    pharos_signal_constants k;
+    pharos_signal_info(c->signal, &k);
+    const double lambda = LIGHT / carrier_hz(k, *c);
+    const double t = (double)(t_rx_ms - s->start_gps_ms) / 1000.0;
+    double wobble = 0.0;
+    if (c->wobble_accel_mps2 != 0.0 && c->wobble_period_s > 0.0) {
+        const double w = 2.0 * M_PI / c->wobble_period_s;
+        wobble = c->wobble_accel_mps2 / (w * w) * sin(w * t);
+    }
+    const double cycles = -(c->doppler_hz * t + 0.5 * c->doppler_rate_hz_s * t * t) + wobble / lambda;
+    const double range = c->range0_m + cycles * lambda;
+    *phase_cycles = cycles;
+    const double d = range / LIGHT * 1000.0;  // travel time in ms
+    const double up = ceil(d);
+    *tx_ms = t_rx_ms - (int64_t)up;
+    *tx_sub_ms = up - d;
+}
+
+int pharos_derive(const pharos_scenario_spec *s, const pharos_channel_spec *c, uint32_t table, uint64_t epoch, pharos_synth_row *out) {
+    pharos_signal_constants k;
+    pharos_signal_info(c->signal, &k);
+    memset(out, 0, sizeof *out);
+    out->table = PHAROS_SYNTH_INACTIVE;
+    const int64_t ta = s->start_gps_ms + (int64_t)epoch, tb = ta + 1;
+    double pa, pb, sa, sb;
+    int64_t ma, mb;
+    pharos_scenario_endpoint(s, c, ta, &pa, &ma, &sa);
+    pharos_scenario_endpoint(s, c, tb, &pb, &mb, &sb);
+    // parameters.derive (parameters.zig:51-70)
+    const double n = (double)s->samples_per_epoch;
+    const int64_t bms = base_ms(k.system, ma, s->leap_seconds);
+    const double span_ms = (double)(mb - ma) + (sb - sa);
+    if (!(fabs(span_ms - 1.0) <= 0.03) || bms < 0) return 0;
+    const int64_t period0 = div_floor(bms, k.code_ms);
+    const double in_period = (double)mod_floor(bms, k.code_ms) + sa;
+    const double epm = (double)k.entries_per_ms;
+    const double phase0 = if_phase(c->if_hz, ta, 0.0) - pa;
+    out->carrier_phase0 = phase0 - floor(phase0);
+    out->carrier_step = (double)c->if_hz / (n * 1000.0) + (pa - pb) / n;
+    out->entry_phase0 = in_period * epm;
+    out->entries_per_sample = span_ms * epm / n;
+    out->period0 = (uint64_t)period0;
+    out->table = table;
+    out->amp = (float)sqrt(2.0 * pow(10.0, c->cn0_dbhz / 10.0) / (1000.0 * n));  // parameters.zig:78-81
+    if (c->lead_component && c->lead_entries > 0.0) {  // channels.zig:353-356
+        out->lead_bits = (uint8_t)(c->lead_component == 1 ? k.state_bits_pilot : k.state_bits_data);
+        out->lead_entries = out->lead_bits ? c->lead_entries : 0.0;
+    }
+    return 1;
+}
+
+uint64_t pharos_scenario_mask_bound(const pharos_scenario_spec *s, const pharos_channel_spec *channels, uint32_t epochs) {
+    uint64_t total = 0;
+    for (uint32_t c = 0; c < s->channel_count; c++) {
+        pharos_signal_constants k;
+        pharos_signal_info(channels[c].signal, &k);
+        // Periods a window of `epochs` ms spans at a 3 % rate deviation, plus the masksNeeded slack.
+        total += (uint64_t)ceil((double)(epochs + 1) * 1.03 / k.code_ms) + 8;
+    }
+    return total;
+}
+
+int64_t pharos_scenario_window(const pharos_scenario_spec *s, const pharos_channel_spec *channels, uint64_t first_epoch, uint32_t epochs,
+                               pharos_synth_row *rows, uint8_t *masks, uint64_t mask_cap) {
+    if (!s || (s->channel_count && !channels) || !rows || epochs == 0) return -2;
+    const uint32_t nch = s->channel_count;
+    uint64_t used = 0;
+    std::vector<uint8_t> sec(4096);
+    for (uint32_t c = 0; c < nch; c++) {
+        const pharos_channel_spec &ch = channels[c];
+        pharos_signal_constants k;
+        if (pharos_signal_info(ch.signal, &k)) return -2;
+        uint64_t first = UINT64_MAX, last = 0;
+        for (uint32_t e = 0; e < epochs; e++) {
+            pharos_synth_row &r = rows[(size_t)e * nch + c];
+            if (!pharos_derive(s, &ch, c, first_epoch + e, &r)) continue;
+            first = std::min<uint64_t>(first, r.period0);
+            last = std::max<uint64_t>(last, r.period0 + masks_needed(r, s->samples_per_epoch, k.entries_per_period));
+        }
+        if (last == 0) continue;
+        const uint64_t len = last - first;
+        if (used + len > mask_cap || len > 65535) return -3;
+        const int sl = pharos_secondary_code(ch.signal, ch.prn, sec.data(), (uint32_t)sec.size());
+        if (sl < 0) return -2;
+        if (masks) fill_masks(k, ch, first, len, sec.data(), (uint32_t)sl, masks + used);
+        for (uint32_t e = 0; e < epochs; e++) {
+            pharos_synth_row &r = rows[(size_t)e * nch + c];
+            if (r.table == PHAROS_SYNTH_INACTIVE) continue;
+            r.mask_offset = (uint32_t)(used + (r.period0 - first));
+            r.mask_count = (uint16_t)(last - r.period0);
+        }
+        used += len;
+    }
+    return (int64_t)used;
+}
+
+}  // extern "C"

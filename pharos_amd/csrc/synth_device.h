@@ -1,0 +1,107 @@
+// Device-side data layout shared by the synthesis kernels and the engine host code.
+#pragma once
+#include <stdint.h>
+#include "../../include/pharos_synth.h"
+
+namespace pharos {
+
+// Launch geometry.  One workgroup synthesises one tile: TILE_BLOCKS rotor blocks of 512 samples of one epoch.
+// Thread (b, l) owns lane l of block b: samples n0 + 512 b + l + 16 k, k < 32 -- the reference's rotor lane
+// (four chains of four lanes advanced together, rotor.zig:5-58), so its f32 recursion is the reference's.
+constexpr uint32_t RESEED = 512;     // rotor.zig:5
+constexpr uint32_t LANES16 = 16;     // rotor.zig:10 (chains x lanes)
+constexpr uint32_t STEPS = RESEED / LANES16;  // 32 advances per block
+constexpr uint32_t TILE_BLOCKS = 8;
+constexpr uint32_t THREADS = TILE_BLOCKS * LANES16;  // 128 = two wave64
+constexpr uint32_t TILE = TILE_BLOCKS * RESEED;      // 4096 samples
+constexpr uint32_t MAX_CHUNK = 16;   // channels staged per LDS pass
+constexpr uint32_t LEAD_ALLOW = 64;  // entries a leading component may be ahead
+constexpr double RATE_MARGIN = 1.031; // derive() admits 3 % (parameters.zig:46,55)
+constexpr uint32_t VT_BYTES = 256;   // value table: 4 period offsets x 8 states x (re, im)
+
+enum Shape : uint32_t { SHAPE_PLAIN = 0, SHAPE_TDM = 1, SHAPE_LEAD = 2 };
+enum Axis : uint32_t { AXIS_BOTH = 0, AXIS_RE = 1, AXIS_IM = 2 };
+
+// A code table on the device.  ext[i] = 8 * states[i % per] + 64 * (i / per) for i < ext_len: one period plus
+// the entries a tile can run past its end, each byte naming its state and its period offset (0..3).
+struct DevTable {
+    const uint8_t *ext;
+    const uint8_t *pilot;      // TDM pilot chips, else null
+    uint32_t per, ext_len;
+    uint32_t count, axis;
+    uint32_t tdm_periods, tdm_half;
+    uint32_t pilot_len, w_max; // w_max: most entries a tile may stage from this table
+    double vre[8], vim[8];
+};
+
+// One engine (one run configuration) on the device.
+struct DevEngine {
+    const DevTable *tables;
+    const double *turn;        // 4096 x (cos, sin), rotor.zig:64-72
+    uint64_t seed;
+    uint64_t bpe;              // bytes per epoch
+    uint32_t ntables, spe, format, noise;
+    uint32_t tiles;            // tiles per epoch
+    uint32_t w_slot;           // LDS staging bytes per channel (multiple of 16)
+    uint32_t chunk;            // channels staged per pass
+    uint32_t lds_bytes;        // dynamic LDS the kernel needs for this engine
+    uint32_t out_bytes;        // LDS bytes of a tile's output units
+    uint32_t pad;
+};
+
+// Per channel, per tile (LDS).
+struct ChanRec {
+    double pw_c[LANES16], pw_s[LANES16];  // w1^l, the rotor's f64 lane powers (rotor.zig:25-30)
+    double c0_c[TILE_BLOCKS], c0_s[TILE_BLOCKS];  // sincos64 of each block's start phase (rotor.zig:37-38)
+    double phase0, step;
+    uint64_t P0;               // Q32.32 code position of the tile's first sample, less `first`
+    uint64_t eps_q;
+    float w_re, w_im;          // f32 sincos64(16 step) (rotor.zig:31-33)
+    uint32_t active, shape, axis, table;
+    uint32_t e0, k_lo, W, lead_q;
+    uint32_t lead_whole, lead_bits, mask_offset, mask_count;
+    uint32_t tdm_pad[2];
+};
+static_assert(sizeof(ChanRec) % 16 == 0, "ChanRec keeps 16-byte alignment");
+
+// Error bits the kernels raise (read back by the host).
+enum : uint32_t {
+    ERR_STAGE = 1u,    // a row needs more staged entries than the tables allow (rate outside +-3 %, lead too long)
+    ERR_MASKS = 2u,    // a row reads a mask beyond mask_count or the mask buffer
+    ERR_TABLE = 4u,    // a row names a table out of range or a zero code rate
+};
+
+// Work description for the persistent batched kernel: one segment = one window of one job.
+struct DevJob {
+    const DevEngine *engine;
+    const pharos_synth_row *rows;
+    const uint8_t *masks;
+    uint8_t *out;
+    uint64_t mask_len;
+    uint64_t first_epoch;
+    uint32_t channels, epochs;
+    uint32_t gain_mode, first_segment;
+    float constant_gain;
+    uint32_t segments;
+};
+
+struct DevSegment {
+    uint64_t first_item;       // first work item of this segment
+    uint32_t job, epoch_off;   // epochs [epoch_off, epoch_off + epochs) of the job
+    uint32_t epochs, tiles;
+};
+
+// AGC state per job on the device (gain.zig:14-35), plus the gains it publishes per segment.
+struct DevAgc {
+    double gain;
+    unsigned long long clipped, components;
+    uint32_t adjustments, pad;
+};
+
+struct DevAdjust {
+    unsigned long long epoch;
+    double gain, rate;
+    uint32_t reduced, pad;
+};
+
+}  // namespace pharos

@@ -1,0 +1,729 @@
+// Engine host code and kernel entry points of pharos_synth.h.
+//
+// Plain form: pharos_synth_window launches one workgroup per (epoch, tile) of a window under one gain -- the
+// GPU replacement of Run.runJob(.produce) fanning synthSubTile out over a thread pool (engine/run.zig:277-296,
+// 367-380, engine/workers.zig:52-71).
+// Batched form: pharos_synth_batch runs a persistent kernel over a queue of (job, window, tile) items; the
+// adaptive gain (synthesis/gain.zig:14-35, applied by engine/run.zig:238-247) is carried on the device.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "synth_kernel.h"
+
+using namespace pharos;
+
+// ------------------------------------------------------------------ errors
+static thread_local char g_error[256];
+
+static void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+}
+
+enum { ST_OK = 0, ST_ARG = 2, ST_NOMEM = 3, ST_INTERNAL = 4 };
+
+#define HIP_TRY(expr, status)                                                         \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            set_error("%s failed: %s", #expr, hipGetErrorString(e_));                 \
+            return (status);                                                          \
+        }                                                                             \
+    } while (0)
+
+extern "C" const char *pharos_synth_last_error(void) { return g_error; }
+extern "C" uint32_t pharos_synth_abi_version(void) { return PHAROS_SYNTH_ABI_VERSION; }
+
+// ------------------------------------------------------------------ kernels
+namespace pharos {
+
+__global__ void clear_words(uint32_t *p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+__global__ __launch_bounds__(THREADS) void synth_window_kernel(const DevEngine *__restrict__ E,
+                                                               const pharos_synth_row *__restrict__ rows,
+                                                               uint32_t channels, const uint8_t *__restrict__ masks,
+                                                               uint64_t mask_len, uint64_t first_epoch, float gain,
+                                                               uint8_t *__restrict__ out, uint32_t *__restrict__ clip_partial,
+                                                               uint32_t *err) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const uint32_t tile = blockIdx.x, e = blockIdx.y;
+    TileArgs A;
+    A.E = E;
+    A.rows = rows + (size_t)e * channels;
+    A.masks = masks;
+    A.mask_len = mask_len;
+    A.channels = channels;
+    A.tile = tile;
+    A.epoch = first_epoch + e;
+    A.out_epoch = out + (size_t)e * E->bpe;
+    uint32_t clips = synth_tile(A, smem, [&]() { return gain; }, err);
+    if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
+}
+
+struct BatchState {
+    const DevJob *jobs;
+    const DevSegment *segs;
+    const uint32_t *seg_next;      // the same job's next segment, or ~0
+    uint32_t nsegs;
+    uint64_t total_items;
+    unsigned long long *queue;     // next work item
+    uint32_t *tiles_done;          // per segment
+    unsigned long long *seg_clip;  // per segment
+    float *seg_gain;               // per segment
+    uint32_t *seg_ready;           // per segment
+    DevAgc *agc;                   // per job
+    DevAdjust *adjust;             // per job, adjust_cap each
+    uint32_t adjust_cap;
+    uint32_t *err;
+};
+
+__device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
+    if (hint < S.nsegs && S.segs[hint].first_item <= item &&
+        (hint + 1 == S.nsegs || S.segs[hint + 1].first_item > item))
+        return hint;
+    if (hint + 1 < S.nsegs && S.segs[hint + 1].first_item <= item &&
+        (hint + 2 == S.nsegs || S.segs[hint + 2].first_item > item))
+        return hint + 1;
+    uint32_t lo = 0, hi = S.nsegs;  // first_item[lo] <= item < first_item[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (S.segs[mid].first_item <= item) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// One AGC step after a window (gain.zig:19-35, accounted as engine/run.zig:240-246 does per window).
+__device__ void agc_account(const BatchState &S, const DevJob &J, uint32_t job, const DevSegment &seg, uint32_t s) {
+    DevAgc &G = S.agc[job];
+    unsigned long long clipped = __hip_atomic_load(&S.seg_clip[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    G.clipped += clipped;
+    G.components += (unsigned long long)seg.epochs * J.engine->spe * 2ull;
+    const unsigned long long epoch_after = J.first_epoch + seg.epoch_off + seg.epochs;
+    if (epoch_after % 100ull == 0) {
+        const double rate = (double)G.clipped / (double)G.components;
+        G.clipped = 0;
+        G.components = 0;
+        int changed = 0, reduced = 0;
+        if (rate > 0.01) { G.gain *= 0.95; changed = 1; reduced = 1; }
+        else if (rate < 0.001 && G.gain < 1.0) { double g = G.gain * 1.02; G.gain = g < 1.0 ? g : 1.0; changed = 1; }
+        if (changed) {
+            if (G.adjustments < S.adjust_cap) {
+                DevAdjust &a = S.adjust[(size_t)job * S.adjust_cap + G.adjustments];
+                a.epoch = epoch_after;
+                a.gain = G.gain;
+                a.rate = rate;
+                a.reduced = (uint32_t)reduced;
+            }
+            G.adjustments++;
+        }
+    }
+    const uint32_t nxt = S.seg_next[s];
+    if (nxt != 0xffffffffu) {
+        S.seg_gain[nxt] = (float)G.gain;
+        __hip_atomic_store(&S.seg_ready[nxt], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void synth_batch_kernel(BatchState S) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ unsigned long long s_item;
+    __shared__ uint32_t s_seg;
+    __shared__ float s_gain;
+    uint32_t hint = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            unsigned long long item = atomicAdd(S.queue, 1ull);
+            s_item = item;
+            if (item < S.total_items) {
+                hint = find_segment(S, item, hint);
+                s_seg = hint;
+            }
+        }
+        __syncthreads();
+        const unsigned long long item = s_item;
+        if (item >= S.total_items) break;
+        const uint32_t s = s_seg;
+        const DevSegment seg = S.segs[s];
+        const DevJob &J = S.jobs[seg.job];
+        const uint64_t local = item - seg.first_item;
+        const uint32_t e_off = seg.epoch_off + (uint32_t)(local / seg.tiles);
+        TileArgs A;
+        A.E = J.engine;
+        A.rows = J.rows + (size_t)e_off * J.channels;
+        A.masks = J.masks;
+        A.mask_len = J.mask_len;
+        A.channels = J.channels;
+        A.tile = (uint32_t)(local % seg.tiles);
+        A.epoch = J.first_epoch + e_off;
+        A.out_epoch = J.out + (size_t)e_off * J.engine->bpe;
+        auto gain_fn = [&]() -> float {
+            if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT) return J.constant_gain;
+            if (threadIdx.x == 0) {
+                while (__hip_atomic_load(&S.seg_ready[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                    __builtin_amdgcn_s_sleep(8);
+                s_gain = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            return s_gain;
+        };
+        const uint32_t clips = synth_tile(A, smem, gain_fn, S.err);
+        if (threadIdx.x == 0) {
+            atomicAdd(&S.seg_clip[s], (unsigned long long)clips);
+            __threadfence();
+            const uint32_t done = atomicAdd(&S.tiles_done[s], 1u) + 1u;
+            if (done == seg.epochs * seg.tiles) {
+                __threadfence();
+                if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) agc_account(S, J, seg.job, seg, s);
+                else {
+                    const uint32_t nxt = S.seg_next[s];
+                    if (nxt != 0xffffffffu) __hip_atomic_store(&S.seg_ready[nxt], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace pharos
+
+// ------------------------------------------------------------------ engine
+struct DeviceBuffer {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct pharos_synth_engine {
+    int device = 0;
+    pharos_synth_config cfg{};
+    DevEngine view{};          // host copy with device pointers
+    DevEngine *d_engine = nullptr;
+    std::vector<void *> owned; // device allocations
+    uint64_t device_bytes = 0;
+    uint32_t *d_err = nullptr;
+    uint32_t *d_clip = nullptr;
+    size_t clip_cap = 0;
+    std::vector<uint32_t> h_clip;
+    std::mutex lock;
+};
+
+static uint32_t bytes_per_sample_x2(uint32_t format) { return format == 0 ? 1 : (format == 4 ? 16u : 2u << (format - 1)); } // in half bytes
+static uint64_t bytes_for_host(uint32_t format, uint64_t samples) { return samples * bytes_per_sample_x2(format) / 2; }
+
+static int dev_alloc(pharos_synth_engine *e, void **p, size_t n) {
+    hipError_t st = hipMalloc(p, n ? n : 16);
+    if (st != hipSuccess) {
+        set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(st));
+        return ST_NOMEM;
+    }
+    e->owned.push_back(*p);
+    e->device_bytes += n;
+    return ST_OK;
+}
+
+static void engine_free(pharos_synth_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    for (void *p : e->owned) (void)hipFree(p);
+    delete e;
+}
+
+// rotor.zig:64-72: the f64 turn table, from the host libm.
+static void build_turn(std::vector<double> &t) {
+    t.resize(2 * 4096);
+    for (int i = 0; i < 4096; i++) {
+        double a = 2.0 * 3.14159265358979323846 * (double)i / 4096.0;
+        t[2 * i] = cos(a);
+        t[2 * i + 1] = sin(a);
+    }
+}
+
+extern "C" int pharos_synth_create(const pharos_synth_config *config, const pharos_synth_table *tables, uint32_t table_count,
+                                   int device, pharos_synth_engine **out) {
+    g_error[0] = 0;
+    if (!out || !config || (table_count && !tables)) { set_error("null argument"); return ST_ARG; }
+    *out = nullptr;
+    const pharos_synth_config &c = *config;
+    if (c.samples_per_epoch < 512 || c.samples_per_epoch > 200000) {
+        set_error("samples_per_epoch must be in [512, 200000], got %u", c.samples_per_epoch);
+        return ST_ARG;
+    }
+    if (c.format > 4) { set_error("format %u is not defined (0 IQ2, 1 IQ4, 2 IQ8, 3 IQ16, 4 F32)", c.format); return ST_ARG; }
+    if (c.noise > 1) { set_error("noise mode %u is not defined (0 counter, 1 off)", c.noise); return ST_ARG; }
+    if (c.format == PHAROS_SYNTH_IQ2 && c.samples_per_epoch % 2) {
+        set_error("IQ2 needs an even number of samples per epoch, got %u", c.samples_per_epoch);
+        return ST_ARG;
+    }
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev), ST_INTERNAL);
+    if (device < 0 || device >= ndev) { set_error("device %d out of range (%d devices)", device, ndev); return ST_ARG; }
+    HIP_TRY(hipSetDevice(device), ST_INTERNAL);
+
+    pharos_synth_engine *e = new (std::nothrow) pharos_synth_engine();
+    if (!e) { set_error("out of memory"); return ST_NOMEM; }
+    e->device = device;
+    e->cfg = c;
+    const uint32_t spe = c.samples_per_epoch;
+    const uint32_t t_eff = std::min<uint32_t>(TILE, (spe + RESEED - 1) / RESEED * RESEED);
+
+    std::vector<DevTable> dt(table_count);
+    uint32_t w_slot_max = 0;
+    for (uint32_t k = 0; k < table_count; k++) {
+        const pharos_synth_table &t = tables[k];
+        if (!t.states || t.entries_per_period == 0 || (t.value_count != 4 && t.value_count != 8) || t.entries_per_ms == 0) {
+            set_error("table %u is incomplete", k);
+            engine_free(e);
+            return ST_ARG;
+        }
+        for (uint32_t i = 0; i < t.entries_per_period; i++) {
+            if (t.states[i] >= t.value_count) {
+                set_error("table %u: state %u at entry %u exceeds its %u values", k, t.states[i], i, t.value_count);
+                engine_free(e);
+                return ST_ARG;
+            }
+        }
+        if (t.tdm_periods && (!t.tdm_pilot || t.tdm_pilot_len == 0 || t.entries_per_period % 2)) {
+            set_error("table %u: TDM table without its pilot code", k);
+            engine_free(e);
+            return ST_ARG;
+        }
+        const uint32_t per = t.entries_per_period;
+        const double eps_max = RATE_MARGIN * (double)t.entries_per_ms / (double)spe;
+        const uint64_t w_max = (uint64_t)ceil((double)t_eff * eps_max) + LEAD_ALLOW + 8;
+        if (w_max > 3ull * per) {
+            set_error("table %u: %u samples per epoch are too few for %u entries per period", k, spe, per);
+            engine_free(e);
+            return ST_ARG;
+        }
+        const uint32_t ext_len = per + (uint32_t)w_max;
+        std::vector<uint8_t> ext(ext_len + 32, 0);
+        for (uint32_t i = 0; i < ext_len; i++) ext[i] = (uint8_t)(8u * t.states[i % per] + 64u * (i / per));
+        DevTable &d = dt[k];
+        memset(&d, 0, sizeof d);
+        void *p;
+        int st;
+        if ((st = dev_alloc(e, &p, ext.size()))) { engine_free(e); return st; }
+        if (hipMemcpy(p, ext.data(), ext.size(), hipMemcpyHostToDevice) != hipSuccess) { set_error("table upload failed"); engine_free(e); return ST_INTERNAL; }
+        d.ext = (const uint8_t *)p;
+        d.per = per;
+        d.ext_len = ext_len;
+        d.count = t.value_count;
+        bool any_re = false, any_im = false;
+        for (int v = 0; v < 8; v++) {
+            d.vre[v] = v < (int)t.value_count ? t.values_re[v] : 0.0;
+            d.vim[v] = v < (int)t.value_count ? t.values_im[v] : 0.0;
+            any_re |= d.vre[v] != 0.0;
+            any_im |= d.vim[v] != 0.0;
+        }
+        d.axis = !any_im ? AXIS_RE : (!any_re ? AXIS_IM : AXIS_BOTH);  // code_tables.zig:89-99
+        d.w_max = (uint32_t)w_max;
+        if (t.tdm_periods) {
+            if ((st = dev_alloc(e, &p, t.tdm_pilot_len))) { engine_free(e); return st; }
+            if (hipMemcpy(p, t.tdm_pilot, t.tdm_pilot_len, hipMemcpyHostToDevice) != hipSuccess) { set_error("pilot upload failed"); engine_free(e); return ST_INTERNAL; }
+            d.pilot = (const uint8_t *)p;
+            d.pilot_len = t.tdm_pilot_len;
+            d.tdm_periods = t.tdm_periods;
+            d.tdm_half = per / 2;
+        }
+        w_slot_max = std::max<uint32_t>(w_slot_max, (uint32_t)w_max + 8);  // staging copies whole dwords from e0 & ~3
+    }
+    void *p;
+    int st;
+    if ((st = dev_alloc(e, &p, sizeof(DevTable) * std::max<uint32_t>(table_count, 1)))) { engine_free(e); return st; }
+    if (table_count && hipMemcpy(p, dt.data(), sizeof(DevTable) * table_count, hipMemcpyHostToDevice) != hipSuccess) { set_error("table upload failed"); engine_free(e); return ST_INTERNAL; }
+    e->view.tables = (const DevTable *)p;
+    std::vector<double> turn;
+    build_turn(turn);
+    if ((st = dev_alloc(e, &p, turn.size() * sizeof(double)))) { engine_free(e); return st; }
+    if (hipMemcpy(p, turn.data(), turn.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { set_error("turn table upload failed"); engine_free(e); return ST_INTERNAL; }
+    e->view.turn = (const double *)p;
+    e->view.seed = c.seed;
+    e->view.bpe = bytes_for_host(c.format, spe);
+    e->view.ntables = table_count;
+    e->view.spe = spe;
+    e->view.format = c.format;
+    e->view.noise = c.noise;
+    e->view.tiles = (spe + TILE - 1) / TILE;
+    e->view.w_slot = (std::max<uint32_t>(w_slot_max, 16) + 15) / 16 * 16;
+    // Channels per LDS pass: as many as fit a 48 KiB budget next to the tile's output units.
+    // Channels per LDS pass: as many as fit a 32 KiB budget; the tile's sums reuse the same bytes afterwards.
+    const uint32_t budget = SUM_BYTES;
+    const uint32_t per_ch = (uint32_t)sizeof(ChanRec) + VT_BYTES + e->view.w_slot;
+    uint32_t chunk = budget / per_ch;
+    chunk = std::max<uint32_t>(1, std::min<uint32_t>(chunk, MAX_CHUNK));
+    e->view.chunk = chunk;
+    e->view.out_bytes = 0;
+    e->view.lds_bytes = std::max<uint32_t>(chunk * per_ch, SUM_BYTES) + 64;
+    if (e->view.lds_bytes > 160 * 1024) {
+        set_error("a channel's code tables need %u bytes of LDS per workgroup", e->view.lds_bytes);
+        engine_free(e);
+        return ST_ARG;
+    }
+    if ((st = dev_alloc(e, &p, sizeof(DevEngine)))) { engine_free(e); return st; }
+    e->d_engine = (DevEngine *)p;
+    if (hipMemcpy(p, &e->view, sizeof(DevEngine), hipMemcpyHostToDevice) != hipSuccess) { set_error("engine upload failed"); engine_free(e); return ST_INTERNAL; }
+    if ((st = dev_alloc(e, &p, 64))) { engine_free(e); return st; }
+    e->d_err = (uint32_t *)p;
+    if (hipMemset(p, 0, 64) != hipSuccess) { set_error("hipMemset failed"); engine_free(e); return ST_INTERNAL; }
+    *out = e;
+    return ST_OK;
+}
+
+extern "C" void pharos_synth_destroy(pharos_synth_engine *e) { engine_free(e); }
+extern "C" uint64_t pharos_synth_bytes_per_epoch(const pharos_synth_engine *e) { return e ? e->view.bpe : 0; }
+extern "C" uint64_t pharos_synth_device_bytes(const pharos_synth_engine *e) { return e ? e->device_bytes : 0; }
+
+// Reads and clears the engine's sticky error word (the stream must be synchronised).
+static int take_device_error(pharos_synth_engine *e, hipStream_t stream) {
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, e->d_err, sizeof err, hipMemcpyDeviceToHost, stream), ST_INTERNAL);
+    HIP_TRY(hipStreamSynchronize(stream), ST_INTERNAL);
+    if (!err) return ST_OK;
+    hipLaunchKernelGGL(clear_words, dim3(1), dim3(64), 0, stream, e->d_err, 16u);
+    HIP_TRY(hipStreamSynchronize(stream), ST_INTERNAL);
+    set_error("signal parameters are outside the supported calculation range (device flags 0x%x: %s%s%s)", err,
+              (err & ERR_STAGE) ? "code rate or lead beyond the staged range; " : "",
+              (err & ERR_MASKS) ? "mask bytes missing; " : "", (err & ERR_TABLE) ? "bad table or code rate" : "");
+    return ST_INTERNAL;
+}
+
+extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+                                   const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
+                                   uint8_t *out, uint64_t *clipped, void *stream_) {
+    g_error[0] = 0;
+    if (!e || !out || (channels && !rows)) { set_error("null argument"); return ST_ARG; }
+    if (epochs == 0) { set_error("epochs must be at least 1"); return ST_ARG; }
+    if (!(gain > 0.0f) || !isfinite(gain)) { set_error("gain must be positive and finite"); return ST_ARG; }
+    std::lock_guard<std::mutex> guard(e->lock);
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(e->device), ST_INTERNAL);
+    const uint32_t tiles = e->view.tiles;
+    const size_t nclip = (size_t)epochs * tiles;
+    if (nclip > e->clip_cap) {
+        if (e->d_clip) {
+            HIP_TRY(hipStreamSynchronize(stream), ST_INTERNAL);
+            (void)hipFree(e->d_clip);
+            e->d_clip = nullptr;
+        }
+        HIP_TRY(hipMalloc((void **)&e->d_clip, nclip * sizeof(uint32_t)), ST_NOMEM);
+        e->clip_cap = nclip;
+    }
+    for (uint32_t done = 0; done < epochs;) {
+        const uint32_t n = std::min<uint32_t>(epochs - done, 65535u);
+        hipLaunchKernelGGL(synth_window_kernel, dim3(tiles, n), dim3(THREADS), e->view.lds_bytes, stream, e->d_engine,
+                           rows + (size_t)done * channels, channels, masks, mask_len, first_epoch + done, gain,
+                           out + (size_t)done * e->view.bpe, e->d_clip + (size_t)done * tiles, e->d_err);
+        HIP_TRY(hipGetLastError(), ST_INTERNAL);
+        done += n;
+    }
+    if (clipped) {
+        e->h_clip.resize(nclip);
+        HIP_TRY(hipMemcpyAsync(e->h_clip.data(), e->d_clip, nclip * sizeof(uint32_t), hipMemcpyDeviceToHost, stream), ST_INTERNAL);
+        int st = take_device_error(e, stream);
+        if (st) return st;
+        for (uint32_t k = 0; k < epochs; k++) {
+            uint64_t s = 0;
+            for (uint32_t t = 0; t < tiles; t++) s += e->h_clip[(size_t)k * tiles + t];
+            clipped[k] = s;
+        }
+    }
+    return ST_OK;
+}
+
+// ------------------------------------------------------------------ batched form
+struct pharos_synth_batch_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<pharos_synth_job> jobs;
+    std::vector<void *> owned;
+    DevAgc *d_agc = nullptr;
+    DevAdjust *d_adjust = nullptr;
+    unsigned long long *d_seg_clip = nullptr;
+    uint32_t nsegs = 0, adjust_cap = 0;
+    std::vector<uint32_t> job_first_seg, job_seg_count;
+    uint32_t *d_err = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<uint8_t> upload;  // host image of the batch tables, alive until the copy has run
+};
+
+static void batch_free(pharos_synth_batch_handle *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->done) (void)hipEventDestroy(h->done);
+    for (void *p : h->owned) (void)hipFree(p);
+    delete h;
+}
+
+static int batch_alloc(pharos_synth_batch_handle *h, void **p, size_t n) {
+    hipError_t st = hipMalloc(p, n ? n : 16);
+    if (st != hipSuccess) { set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(st)); return ST_NOMEM; }
+    h->owned.push_back(*p);
+    return ST_OK;
+}
+
+static uint32_t g_cu_count[64];
+
+static uint32_t cu_count(int device) {
+    if (device >= 0 && device < 64 && g_cu_count[device]) return g_cu_count[device];
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+    if (device >= 0 && device < 64) g_cu_count[device] = (uint32_t)n;
+    return (uint32_t)n;
+}
+
+extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning,
+                                         void *stream_, pharos_synth_batch_handle **out) {
+    g_error[0] = 0;
+    if (!out || (njobs && !jobs)) { set_error("null argument"); return ST_ARG; }
+    *out = nullptr;
+    if (njobs == 0) { set_error("a batch needs at least one job"); return ST_ARG; }
+    const int device = jobs[0].engine ? jobs[0].engine->device : 0;
+    uint32_t lds = 0;
+    for (uint32_t j = 0; j < njobs; j++) {
+        const pharos_synth_job &J = jobs[j];
+        if (!J.engine || !J.out || (J.channels && !J.rows) || J.epochs == 0) { set_error("job %u is incomplete", j); return ST_ARG; }
+        if (J.engine->device != device) { set_error("job %u runs on another device", j); return ST_ARG; }
+        if (J.gain_mode > 1) { set_error("job %u: gain mode %u is not defined", j, J.gain_mode); return ST_ARG; }
+        if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT && !(J.constant_gain > 0.0f)) { set_error("job %u: constant gain must be positive", j); return ST_ARG; }
+        if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE && !J.agc) { set_error("job %u: adaptive gain needs its AGC state", j); return ST_ARG; }
+        lds = std::max(lds, J.engine->view.lds_bytes);
+    }
+    HIP_TRY(hipSetDevice(device), ST_INTERNAL);
+    pharos_synth_batch_handle *h = new (std::nothrow) pharos_synth_batch_handle();
+    if (!h) { set_error("out of memory"); return ST_NOMEM; }
+    h->device = device;
+    h->stream = (hipStream_t)stream_;
+    h->jobs.assign(jobs, jobs + njobs);
+
+    // Windows end where the AGC evaluates (multiples of 100 epochs, gain.zig:5,90-93).  Items run round by round:
+    // every job's first window, then every job's second, so a window's gain is ready long before it is needed.
+    struct Win { uint32_t job, off, epochs; };
+    std::vector<std::vector<Win>> per_job(njobs);
+    size_t rounds = 0;
+    for (uint32_t j = 0; j < njobs; j++) {
+        const pharos_synth_job &J = jobs[j];
+        uint32_t off = 0;
+        while (off < J.epochs) {
+            const uint64_t abs_e = J.first_epoch + off;
+            const uint32_t to_eval = (uint32_t)(100 - abs_e % 100);
+            const uint32_t n = std::min<uint32_t>(to_eval, J.epochs - off);
+            per_job[j].push_back({j, off, n});
+            off += n;
+        }
+        rounds = std::max(rounds, per_job[j].size());
+    }
+    std::vector<DevSegment> segs;
+    std::vector<uint32_t> seg_next;
+    std::vector<uint32_t> last_seg(njobs, 0xffffffffu);
+    h->job_first_seg.assign(njobs, 0xffffffffu);
+    h->job_seg_count.assign(njobs, 0);
+    uint64_t items = 0;
+    for (size_t r = 0; r < rounds; r++) {
+        for (uint32_t j = 0; j < njobs; j++) {
+            if (r >= per_job[j].size()) continue;
+            const Win &w = per_job[j][r];
+            DevSegment s;
+            s.first_item = items;
+            s.job = j;
+            s.epoch_off = w.off;
+            s.epochs = w.epochs;
+            s.tiles = jobs[j].engine->view.tiles;
+            items += (uint64_t)s.epochs * s.tiles;
+            const uint32_t idx = (uint32_t)segs.size();
+            segs.push_back(s);
+            seg_next.push_back(0xffffffffu);
+            if (last_seg[j] != 0xffffffffu) seg_next[last_seg[j]] = idx;
+            else h->job_first_seg[j] = idx;
+            last_seg[j] = idx;
+            h->job_seg_count[j]++;
+        }
+    }
+    const uint32_t nsegs = (uint32_t)segs.size();
+    h->nsegs = nsegs;
+    uint32_t adjust_cap = 1;
+    for (uint32_t j = 0; j < njobs; j++) adjust_cap = std::max(adjust_cap, jobs[j].adjustments_cap);
+    h->adjust_cap = adjust_cap;
+
+    std::vector<DevJob> dj(njobs);
+    std::vector<DevAgc> agc(njobs);
+    std::vector<float> seg_gain(nsegs, 1.0f);
+    std::vector<uint32_t> seg_ready(nsegs, 0);
+    for (uint32_t j = 0; j < njobs; j++) {
+        const pharos_synth_job &J = jobs[j];
+        DevJob &d = dj[j];
+        memset(&d, 0, sizeof d);
+        d.engine = J.engine->d_engine;
+        d.rows = J.rows;
+        d.masks = J.masks;
+        d.out = J.out;
+        d.mask_len = J.mask_len;
+        d.first_epoch = J.first_epoch;
+        d.channels = J.channels;
+        d.epochs = J.epochs;
+        d.gain_mode = J.gain_mode;
+        d.constant_gain = J.constant_gain;
+        memset(&agc[j], 0, sizeof(DevAgc));
+        if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) {
+            agc[j].gain = J.agc->gain;
+            agc[j].clipped = J.agc->clipped;
+            agc[j].components = J.agc->components;
+        }
+        const uint32_t f = h->job_first_seg[j];
+        seg_gain[f] = J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE ? (float)J.agc->gain : J.constant_gain;
+        seg_ready[f] = 1;
+    }
+    // Device buffers: one allocation carved into the pieces.
+    struct Piece { size_t off, size; };
+    size_t total = 0;
+    auto carve = [&](size_t n) { size_t o = (total + 255) & ~(size_t)255; total = o + n; return Piece{o, n}; };
+    Piece p_jobs = carve(sizeof(DevJob) * njobs), p_segs = carve(sizeof(DevSegment) * nsegs),
+          p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long)),
+          p_done = carve(sizeof(uint32_t) * nsegs), p_clip = carve(sizeof(unsigned long long) * nsegs),
+          p_gain = carve(sizeof(float) * nsegs), p_ready = carve(sizeof(uint32_t) * nsegs),
+          p_agc = carve(sizeof(DevAgc) * njobs), p_adj = carve(sizeof(DevAdjust) * (size_t)adjust_cap * njobs),
+          p_err = carve(64);
+    std::vector<uint8_t> host(total, 0);
+    memcpy(host.data() + p_jobs.off, dj.data(), p_jobs.size);
+    memcpy(host.data() + p_segs.off, segs.data(), p_segs.size);
+    memcpy(host.data() + p_next.off, seg_next.data(), p_next.size);
+    memcpy(host.data() + p_gain.off, seg_gain.data(), p_gain.size);
+    memcpy(host.data() + p_ready.off, seg_ready.data(), p_ready.size);
+    memcpy(host.data() + p_agc.off, agc.data(), p_agc.size);
+    void *base;
+    int st = batch_alloc(h, &base, total);
+    if (st) { batch_free(h); return st; }
+    uint8_t *b8 = (uint8_t *)base;
+    h->upload = std::move(host);
+    if (hipMemcpyAsync(base, h->upload.data(), total, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+        set_error("batch upload failed");
+        batch_free(h);
+        return ST_INTERNAL;
+    }
+    BatchState S;
+    S.jobs = (const DevJob *)(b8 + p_jobs.off);
+    S.segs = (const DevSegment *)(b8 + p_segs.off);
+    S.seg_next = (const uint32_t *)(b8 + p_next.off);
+    S.nsegs = nsegs;
+    S.total_items = items;
+    S.queue = (unsigned long long *)(b8 + p_queue.off);
+    S.tiles_done = (uint32_t *)(b8 + p_done.off);
+    S.seg_clip = (unsigned long long *)(b8 + p_clip.off);
+    S.seg_gain = (float *)(b8 + p_gain.off);
+    S.seg_ready = (uint32_t *)(b8 + p_ready.off);
+    S.agc = (DevAgc *)(b8 + p_agc.off);
+    S.adjust = (DevAdjust *)(b8 + p_adj.off);
+    S.adjust_cap = adjust_cap;
+    S.err = (uint32_t *)(b8 + p_err.off);
+    h->d_agc = S.agc;
+    h->d_adjust = S.adjust;
+    h->d_seg_clip = S.seg_clip;
+    h->d_err = S.err;
+
+    uint32_t per_cu = tuning && tuning->workgroups_per_cu ? tuning->workgroups_per_cu : 0;
+    if (!per_cu) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, synth_batch_kernel, THREADS, lds) != hipSuccess || occ <= 0) occ = 2;
+        per_cu = (uint32_t)occ;
+    }
+    uint64_t grid = (uint64_t)cu_count(device) * per_cu;
+    if (grid > items) grid = items;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(synth_batch_kernel, dim3((uint32_t)grid), dim3(THREADS), lds, h->stream, S);
+    if (hipGetLastError() != hipSuccess) { set_error("batch launch failed"); batch_free(h); return ST_INTERNAL; }
+    if (hipEventCreateWithFlags(&h->done, hipEventDisableTiming) != hipSuccess || hipEventRecord(h->done, h->stream) != hipSuccess) {
+        set_error("event record failed");
+        (void)hipStreamSynchronize(h->stream);
+        batch_free(h);
+        return ST_INTERNAL;
+    }
+    *out = h;
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_batch_query(pharos_synth_batch_handle *h) {
+    if (!h) return -ST_ARG;
+    hipError_t st = hipEventQuery(h->done);
+    if (st == hipSuccess) return 1;
+    if (st == hipErrorNotReady) return 0;
+    set_error("batch failed: %s", hipGetErrorString(st));
+    return -ST_INTERNAL;
+}
+
+extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
+    if (!h) { set_error("null handle"); return ST_ARG; }
+    (void)hipSetDevice(h->device);
+    const uint32_t njobs = (uint32_t)h->jobs.size();
+    std::vector<DevAgc> agc(njobs);
+    std::vector<DevAdjust> adj((size_t)h->adjust_cap * njobs);
+    std::vector<unsigned long long> seg_clip(h->nsegs);
+    uint32_t err = 0;
+    int status = ST_OK;
+    if (hipEventSynchronize(h->done) != hipSuccess ||
+        hipMemcpy(agc.data(), h->d_agc, sizeof(DevAgc) * njobs, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(adj.data(), h->d_adjust, sizeof(DevAdjust) * adj.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(seg_clip.data(), h->d_seg_clip, sizeof(unsigned long long) * h->nsegs, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&err, h->d_err, sizeof err, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("batch failed: %s", hipGetErrorString(hipGetLastError()));
+        batch_free(h);
+        return ST_INTERNAL;
+    }
+    // Segments of job j are job_first_seg[j] then each seg_next; sum their clip counts by walking the rounds.
+    std::vector<uint64_t> job_clip(njobs, 0);
+    {
+        // Rebuild the job of every segment in submission order (round-major, as built at submit).
+        std::vector<uint32_t> remaining = h->job_seg_count;
+        uint32_t s = 0;
+        while (s < h->nsegs) {
+            for (uint32_t j = 0; j < njobs && s < h->nsegs; j++) {
+                if (!remaining[j]) continue;
+                remaining[j]--;
+                job_clip[j] += seg_clip[s++];
+            }
+        }
+    }
+    for (uint32_t j = 0; j < njobs; j++) {
+        const pharos_synth_job &J = h->jobs[j];
+        if (J.clipped) *J.clipped = job_clip[j];
+        if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE && J.agc) {
+            const uint32_t prior = J.agc->adjustments;
+            J.agc->gain = agc[j].gain;
+            J.agc->clipped = agc[j].clipped;
+            J.agc->components = agc[j].components;
+            J.agc->adjustments = prior + agc[j].adjustments;
+            if (J.adjustments) {
+                const uint32_t n = std::min(agc[j].adjustments, std::min(J.adjustments_cap, h->adjust_cap));
+                for (uint32_t k = 0; k < n; k++) {
+                    const DevAdjust &a = adj[(size_t)j * h->adjust_cap + k];
+                    J.adjustments[k].epoch = a.epoch;
+                    J.adjustments[k].gain = a.gain;
+                    J.adjustments[k].clip_rate = a.rate;
+                    J.adjustments[k].reduced = a.reduced;
+                    J.adjustments[k].reserved = 0;
+                }
+            }
+        }
+    }
+    if (err) {
+        set_error("signal parameters are outside the supported calculation range (device flags 0x%x)", err);
+        status = ST_INTERNAL;
+    }
+    batch_free(h);
+    return status;
+}
+
+extern "C" int pharos_synth_batch(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning, void *stream) {
+    pharos_synth_batch_handle *h = nullptr;
+    int st = pharos_synth_batch_submit(jobs, njobs, tuning, stream, &h);
+    if (st) return st;
+    return pharos_synth_batch_wait(h);
+}
