@@ -157,6 +157,26 @@ int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t job_count, 
 int pharos_synth_batch_query(pharos_synth_batch_handle *handle); /* 1 done, 0 running, <0 error */
 int pharos_synth_batch_wait(pharos_synth_batch_handle *handle);  /* frees the handle */
 
+
+/* Device plumbing for callers without a HIP runtime of their own (the library's runtime owns the device).
+ * Copies are synchronous; kind 0 host to device, 1 device to host, 2 device to device. */
+int pharos_synth_device_count(void);
+int pharos_synth_mem_info(int device, uint64_t *free_bytes, uint64_t *total_bytes);
+int pharos_synth_malloc(int device, uint64_t bytes, void **out);
+int pharos_synth_free(void *ptr);
+int pharos_synth_memcpy(void *dst, const void *src, uint64_t bytes, int kind);
+int pharos_synth_host_alloc(uint64_t bytes, void **out); /* pinned host memory */
+int pharos_synth_host_free(void *ptr);
+int pharos_synth_stream_create(int device, void **out);
+int pharos_synth_stream_destroy(void *stream);
+int pharos_synth_stream_sync(void *stream);
+int pharos_synth_memcpy_async(void *dst, const void *src, uint64_t bytes, int kind, void *stream);
+int pharos_synth_event_create(void **out);
+int pharos_synth_event_destroy(void *event);
+int pharos_synth_event_record(void *event, void *stream);
+int pharos_synth_event_elapsed_ms(void *start, void *end, float *ms); /* waits for end */
+int pharos_synth_device_sync(int device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,22 @@ def lib():
         "pharos_synth_batch_submit": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP, P(VP)]),
         "pharos_synth_batch_query": (C.c_int, [VP]),
         "pharos_synth_batch_wait": (C.c_int, [VP]),
+        "pharos_synth_device_count": (C.c_int, []),
+        "pharos_synth_mem_info": (C.c_int, [C.c_int, P(U64), P(U64)]),
+        "pharos_synth_malloc": (C.c_int, [C.c_int, U64, P(VP)]),
+        "pharos_synth_free": (C.c_int, [VP]),
+        "pharos_synth_memcpy": (C.c_int, [VP, VP, U64, C.c_int]),
+        "pharos_synth_memcpy_async": (C.c_int, [VP, VP, U64, C.c_int, VP]),
+        "pharos_synth_host_alloc": (C.c_int, [U64, P(VP)]),
+        "pharos_synth_host_free": (C.c_int, [VP]),
+        "pharos_synth_stream_create": (C.c_int, [C.c_int, P(VP)]),
+        "pharos_synth_stream_destroy": (C.c_int, [VP]),
+        "pharos_synth_stream_sync": (C.c_int, [VP]),
+        "pharos_synth_event_create": (C.c_int, [P(VP)]),
+        "pharos_synth_event_destroy": (C.c_int, [VP]),
+        "pharos_synth_event_record": (C.c_int, [VP, VP]),
+        "pharos_synth_event_elapsed_ms": (C.c_int, [VP, VP, P(C.c_float)]),
+        "pharos_synth_device_sync": (C.c_int, [C.c_int]),
         "pharos_host_set_data_dir": (C.c_int, [C.c_char_p]),
         "pharos_signal_info": (C.c_int, [U32, P(SignalConstants)]),
         "pharos_primary_code": (C.c_int, [U32, U32, C.c_int, P(C.c_uint8), U32]),

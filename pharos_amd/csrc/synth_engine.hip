@@ -727,3 +727,109 @@ extern "C" int pharos_synth_batch(const pharos_synth_job *jobs, uint32_t njobs, 
     if (st) return st;
     return pharos_synth_batch_wait(h);
 }
+
+// ------------------------------------------------------------------ device plumbing
+extern "C" int pharos_synth_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int pharos_synth_mem_info(int device, uint64_t *free_bytes, uint64_t *total_bytes) {
+    HIP_TRY(hipSetDevice(device), ST_ARG);
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t), ST_INTERNAL);
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_malloc(int device, uint64_t bytes, void **out) {
+    if (!out) { set_error("null argument"); return ST_ARG; }
+    HIP_TRY(hipSetDevice(device), ST_ARG);
+    HIP_TRY(hipMalloc(out, bytes ? bytes : 16), ST_NOMEM);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_free(void *ptr) {
+    if (ptr) HIP_TRY(hipFree(ptr), ST_INTERNAL);
+    return ST_OK;
+}
+
+static hipMemcpyKind copy_kind(int kind) {
+    return kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+}
+
+extern "C" int pharos_synth_memcpy(void *dst, const void *src, uint64_t bytes, int kind) {
+    if (!bytes) return ST_OK;
+    if (!dst || !src || kind < 0 || kind > 2) { set_error("bad copy arguments"); return ST_ARG; }
+    HIP_TRY(hipMemcpy(dst, src, bytes, copy_kind(kind)), ST_INTERNAL);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_memcpy_async(void *dst, const void *src, uint64_t bytes, int kind, void *stream) {
+    if (!bytes) return ST_OK;
+    if (!dst || !src || kind < 0 || kind > 2) { set_error("bad copy arguments"); return ST_ARG; }
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, copy_kind(kind), (hipStream_t)stream), ST_INTERNAL);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_host_alloc(uint64_t bytes, void **out) {
+    if (!out) { set_error("null argument"); return ST_ARG; }
+    HIP_TRY(hipHostMalloc(out, bytes ? bytes : 16, hipHostMallocDefault), ST_NOMEM);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_host_free(void *ptr) {
+    if (ptr) HIP_TRY(hipHostFree(ptr), ST_INTERNAL);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_stream_create(int device, void **out) {
+    if (!out) { set_error("null argument"); return ST_ARG; }
+    HIP_TRY(hipSetDevice(device), ST_ARG);
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), ST_INTERNAL);
+    *out = s;
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_stream_destroy(void *stream) {
+    if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream), ST_INTERNAL);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_stream_sync(void *stream) {
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream), ST_INTERNAL);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_event_create(void **out) {
+    if (!out) { set_error("null argument"); return ST_ARG; }
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e), ST_INTERNAL);
+    *out = e;
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_event_destroy(void *event) {
+    if (event) HIP_TRY(hipEventDestroy((hipEvent_t)event), ST_INTERNAL);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_event_record(void *event, void *stream) {
+    HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), ST_INTERNAL);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_event_elapsed_ms(void *start, void *end, float *ms) {
+    HIP_TRY(hipEventSynchronize((hipEvent_t)end), ST_INTERNAL);
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end), ST_INTERNAL);
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_device_sync(int device) {
+    HIP_TRY(hipSetDevice(device), ST_ARG);
+    HIP_TRY(hipDeviceSynchronize(), ST_INTERNAL);
+    return ST_OK;
+}

@@ -1,27 +1,108 @@
-"""The HIP engine from Python: device buffers (torch on ROCm) around the pharos_synth C ABI.
+"""The HIP engine from Python, over the pharos_synth C ABI.
 
-torch provides device memory and streams only; every sample is computed by libpharos_amd.so's gfx950 kernels.
+Device memory, streams and events come from the library itself (pharos_synth_malloc and friends), so one HIP
+runtime owns the device; every sample is computed by libpharos_amd.so's gfx950 kernels.
 """
 import ctypes as C
 
 import numpy as np
 
 from . import _lib
-from ._lib import check, ROW_DTYPE
+from ._lib import check
 
 
-def _torch():
-    import torch
-    if not torch.cuda.is_available():
+def device_count():
+    return int(_lib.lib().pharos_synth_device_count())
+
+
+def require_gpu():
+    if device_count() < 1:
         raise RuntimeError("no ROCm GPU visible: the pharos engine runs on the GPU only")
-    return torch
+
+
+class DeviceBuffer:
+    """A block of device memory owned by the library's HIP runtime."""
+
+    def __init__(self, nbytes, device=0):
+        self.nbytes = int(nbytes)
+        self.device = device
+        p = C.c_void_p()
+        check(_lib.lib().pharos_synth_malloc(device, max(self.nbytes, 16), C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, array, device=0):
+        a = np.ascontiguousarray(array)
+        buf = cls(a.nbytes, device)
+        if a.nbytes:
+            check(_lib.lib().pharos_synth_memcpy(buf.ptr, a.ctypes.data, a.nbytes, 0))
+        return buf
+
+    def at(self, offset):
+        """Device address `offset` bytes in."""
+        return self.ptr + int(offset)
+
+    def to_numpy(self, nbytes=None, offset=0):
+        n = self.nbytes - offset if nbytes is None else int(nbytes)
+        out = np.empty(n, dtype=np.uint8)
+        if n:
+            check(_lib.lib().pharos_synth_memcpy(out.ctypes.data, self.at(offset), n, 1))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            _lib.lib().pharos_synth_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self, device=0):
+        s = C.c_void_p()
+        check(_lib.lib().pharos_synth_stream_create(device, C.byref(s)))
+        self.handle = s.value
+
+    def sync(self):
+        check(_lib.lib().pharos_synth_stream_sync(self.handle))
+
+    def __del__(self):
+        try:
+            _lib.lib().pharos_synth_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(_lib.lib().pharos_synth_event_create(C.byref(e)))
+        self.handle = e.value
+
+    def record(self, stream=None):
+        check(_lib.lib().pharos_synth_event_record(self.handle, stream.handle if stream else None))
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        check(_lib.lib().pharos_synth_event_elapsed_ms(self.handle, end.handle, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            _lib.lib().pharos_synth_event_destroy(self.handle)
+        except Exception:
+            pass
 
 
 class Engine:
     """pharos_synth_engine for one scenario configuration (its tables, rate, format, noise and seed)."""
 
     def __init__(self, scenario, device=0, format_override=None):
-        torch = _torch()
+        require_gpu()
         L = _lib.lib()
         self.scenario = scenario
         self.device = device
@@ -30,7 +111,6 @@ class Engine:
         cfg = _lib.SynthConfig(scenario.samples_per_epoch, self.format, 0 if scenario.noise else 1, 0,
                                scenario.seed & (2**64 - 1))
         h = C.c_void_p()
-        torch.cuda.set_device(device)
         check(L.pharos_synth_create(C.byref(cfg), tabs.array, tabs.count, device, C.byref(h)))
         self.handle = h
         self.bytes_per_epoch = int(L.pharos_synth_bytes_per_epoch(h))
@@ -50,50 +130,39 @@ class Engine:
     def device_bytes(self):
         return int(_lib.lib().pharos_synth_device_bytes(self.handle))
 
-    def window(self, first_epoch, epochs, rows_dev, masks_dev, mask_len, gain, out_dev, clips=True, stream=None):
-        """Plain launch of `epochs` epochs; returns per-epoch clip counts (numpy) or None."""
-        torch = _torch()
+    def window(self, first_epoch, epochs, rows_ptr, masks_ptr, mask_len, gain, out_ptr, clips=True, stream=None):
+        """Plain launch of `epochs` epochs (device pointers); returns per-epoch clip counts or None."""
         L = _lib.lib()
         nch = len(self.scenario.channels)
         clipped = (C.c_uint64 * epochs)() if clips else None
-        s = stream.cuda_stream if stream is not None else torch.cuda.current_stream(self.device).cuda_stream
-        check(L.pharos_synth_window(self.handle, first_epoch, epochs, nch, rows_dev.data_ptr() if nch else None,
-                                    masks_dev.data_ptr(), mask_len, float(gain), out_dev.data_ptr(), clipped, s))
+        check(L.pharos_synth_window(self.handle, first_epoch, epochs, nch, rows_ptr if nch else None, masks_ptr,
+                                    mask_len, float(gain), out_ptr, clipped, stream.handle if stream else None))
         return np.frombuffer(clipped, dtype=np.uint64).copy() if clips else None
 
 
-def upload(array, device=0):
-    """A numpy array's bytes as a device uint8 tensor."""
-    torch = _torch()
-    a = np.ascontiguousarray(array)
-    t = torch.from_numpy(a.view(np.uint8).reshape(-1).copy())
-    return t.to(f"cuda:{device}")
-
-
 def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=None, window=100, gain=None):
-    """Synthesises epochs with per-window plain launches and the host AGC between windows (the plain per-run form).
+    """Per-window plain launches with the host AGC between windows (gain.zig:19-35, engine/run.zig:238-247).
 
-    Returns (bytes as numpy uint8, clip counts per epoch, final AGC gain)."""
-    torch = _torch()
+    Returns (bytes, per-epoch clips, the AGC's gain after the run, its adjustments as (epoch, gain, rate))."""
     epochs = scenario.epochs if epochs is None else epochs
     eng = Engine(scenario, device, format_override)
     rows, masks = scenario.window(first_epoch, epochs)
-    d_rows, d_masks = upload(rows, device), upload(masks, device)
-    out = torch.empty(epochs * eng.bytes_per_epoch, dtype=torch.uint8, device=f"cuda:{device}")
+    d_rows, d_masks = DeviceBuffer.from_array(rows, device), DeviceBuffer.from_array(masks, device)
+    out = DeviceBuffer(epochs * eng.bytes_per_epoch, device)
     nch = len(scenario.channels)
-    agc_gain = 1.0
     constant = scenario.gain_db is not None or gain is not None
-    g = gain if gain is not None else (float(np.float32(10.0 ** (scenario.gain_db / 20.0))) if scenario.gain_db is not None else 1.0)
+    g = gain if gain is not None else (float(np.float32(10.0 ** (scenario.gain_db / 20.0)))
+                                       if scenario.gain_db is not None else 1.0)
+    agc_gain, acc_clip, acc_comp, adjustments = 1.0, 0, 0, []
     clips = np.zeros(epochs, dtype=np.uint64)
-    acc_clip, acc_comp = 0, 0
     e = 0
     while e < epochs:
         absolute = first_epoch + e
         n = min(window - absolute % window, epochs - e)
         gw = g if constant else float(np.float32(agc_gain))
-        c = eng.window(absolute, n, d_rows[e * nch * 64:], d_masks, len(masks), gw, out[e * eng.bytes_per_epoch:])
+        c = eng.window(absolute, n, d_rows.at(e * nch * 64), d_masks.ptr, len(masks), gw, out.at(e * eng.bytes_per_epoch))
         clips[e:e + n] = c
-        if not constant:  # gain.zig:19-35
+        if not constant:
             acc_clip += int(c.sum())
             acc_comp += n * scenario.samples_per_epoch * 2
             if (absolute + n) % 100 == 0:
@@ -101,10 +170,11 @@ def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=No
                 acc_clip = acc_comp = 0
                 if rate > 0.01:
                     agc_gain *= 0.95
+                    adjustments.append((absolute + n, agc_gain, rate))
                 elif rate < 0.001 and agc_gain < 1.0:
                     agc_gain = min(agc_gain * 1.02, 1.0)
+                    adjustments.append((absolute + n, agc_gain, rate))
         e += n
-    torch.cuda.synchronize(device)
-    result = out.cpu().numpy()
+    result = out.to_numpy()
     eng.close()
-    return result, clips, agc_gain
+    return result, clips, agc_gain, adjustments

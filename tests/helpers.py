@@ -25,22 +25,21 @@ def oracle_window(scenario, first_epoch, epochs, gain=1.0, threads=8, rows=None,
 
 def gpu_window(scenario, first_epoch, epochs, gain=1.0, sums=True, rows=None, masks=None):
     """GPU bytes and clips of a window (plain launch), and its pre-quantisation sums from the F32 format."""
-    import torch
-    from pharos_amd.engine import Engine, upload
+    from pharos_amd.engine import Engine, DeviceBuffer
     if rows is None:
         rows, masks = scenario.window(first_epoch, epochs)
     eng = Engine(scenario)
-    d_rows, d_masks = upload(rows), upload(masks)
-    out = torch.empty(epochs * eng.bytes_per_epoch, dtype=torch.uint8, device="cuda")
-    clips = eng.window(first_epoch, epochs, d_rows, d_masks, len(masks), gain, out)
-    result = out.cpu().numpy()
+    d_rows, d_masks = DeviceBuffer.from_array(rows), DeviceBuffer.from_array(masks)
+    out = DeviceBuffer(epochs * eng.bytes_per_epoch)
+    clips = eng.window(first_epoch, epochs, d_rows.ptr, d_masks.ptr, len(masks), gain, out.ptr)
+    result = out.to_numpy()
     eng.close()
     sre = sim = None
     if sums:
         eng32 = Engine(scenario, format_override=F32)
-        out32 = torch.empty(epochs * eng32.bytes_per_epoch, dtype=torch.uint8, device="cuda")
-        eng32.window(first_epoch, epochs, d_rows, d_masks, len(masks), gain, out32)
-        pairs = out32.cpu().numpy().view(np.float32).reshape(-1, 2)
+        out32 = DeviceBuffer(epochs * eng32.bytes_per_epoch)
+        eng32.window(first_epoch, epochs, d_rows.ptr, d_masks.ptr, len(masks), gain, out32.ptr)
+        pairs = out32.to_numpy().view(np.float32).reshape(-1, 2)
         sre, sim = pairs[:, 0].copy(), pairs[:, 1].copy()
         eng32.close()
     return result, clips, sre, sim
