@@ -178,3 +178,50 @@ def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=No
     result = out.to_numpy()
     eng.close()
     return result, clips, agc_gain, adjustments
+
+
+class Job:
+    """One run in a batch: its engine, device rows/masks/output and its gain state."""
+
+    def __init__(self, scenario, first_epoch=0, epochs=None, device=0, rows=None, masks=None, adjustments_cap=4096):
+        self.scenario = scenario
+        self.first_epoch = first_epoch
+        self.epochs = scenario.epochs if epochs is None else epochs
+        self.engine = Engine(scenario, device)
+        if rows is None:
+            rows, masks = scenario.window(first_epoch, self.epochs)
+        self.rows = DeviceBuffer.from_array(rows, device)
+        self.masks = DeviceBuffer.from_array(masks, device)
+        self.mask_len = len(masks)
+        self.out = DeviceBuffer(self.epochs * self.engine.bytes_per_epoch, device)
+        self.agc = _lib.SynthAgc(1.0, 0, 0, 0, 0)
+        self.adjust = (_lib.SynthAdjustment * adjustments_cap)()
+        self.clipped = C.c_uint64(0)
+
+    @property
+    def constant_gain(self):
+        g = self.scenario.gain_db
+        return None if g is None else float(np.float32(10.0 ** (g / 20.0)))
+
+    def record(self):
+        cg = self.constant_gain
+        return _lib.SynthJob(self.engine.handle.value, self.first_epoch, self.epochs, len(self.scenario.channels),
+                             self.rows.ptr, self.masks.ptr, self.mask_len, self.out.ptr,
+                             1 if cg is not None else 0, cg if cg is not None else 0.0, C.pointer(self.agc),
+                             C.cast(self.adjust, C.POINTER(_lib.SynthAdjustment)), len(self.adjust), 0,
+                             C.pointer(self.clipped))
+
+    def adjustments(self):
+        n = min(self.agc.adjustments, len(self.adjust))
+        return [(int(a.epoch), float(a.gain), float(a.clip_rate)) for a in self.adjust[:n]]
+
+    def result(self):
+        return self.out.to_numpy()
+
+
+def run_batch(jobs, workgroups_per_cu=0, stream=None):
+    """Runs jobs (a list of Job) in one persistent launch and waits for it."""
+    arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
+    tuning = _lib.SynthTuning(workgroups_per_cu, (C.c_uint32 * 3)())
+    check(_lib.lib().pharos_synth_batch(arr, len(jobs), C.byref(tuning), stream.handle if stream else None))
+    return jobs

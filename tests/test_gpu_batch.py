@@ -1,0 +1,63 @@
+"""The batched form (persistent kernel, device AGC) against the plain form (per-window launches, host AGC).
+
+Each job's bytes, clip count and AGC adjustments must be the same alone and in any mix (the reference's
+determinism contract, docs/design/engine.md:235-243), and the device AGC must follow gain.zig:19-35 exactly.
+"""
+import numpy as np
+import pytest
+
+from pharos_amd import scenario as S
+from pharos_amd.scenario import Scenario, Channel
+from pharos_amd.engine import Job, run_batch, run_plain
+
+pytestmark = pytest.mark.gpu
+
+
+def _strong_iq4(seed, spe=16_368, epochs=700, first=0):
+    r = np.random.default_rng(seed)
+    chans = [Channel("L1CA", int(p), if_hz=int(r.integers(-3_000_000, 3_000_000)), doppler_hz=float(r.uniform(-4000, 4000)),
+                     cn0_dbhz=float(r.uniform(58, 62)), nav_seed=int(r.integers(1, 2**62)))
+             for p in r.choice(np.arange(1, 33), size=20, replace=False)]
+    return Scenario("strong_l1ca_iq4", spe, "IQ4", epochs, chans, noise=True, seed=seed, gain_db=None)
+
+
+def test_batch_equals_plain_constant_gain():
+    sc = S.c0(seed=11, epochs=250)
+    print("case c0 constant gain, 250 epochs", flush=True)
+    pb, pc, _, _ = run_plain(sc)
+    (job,) = run_batch([Job(sc)])
+    assert np.array_equal(job.result(), pb)
+    assert job.clipped.value == int(pc.sum())
+
+
+def test_device_agc_follows_the_host_agc():
+    sc = _strong_iq4(5)
+    print("case strong IQ4 adaptive gain, 700 epochs", flush=True)
+    pb, pc, pgain, padj = run_plain(sc)
+    assert len(padj) >= 3, "the scenario must clip enough to move the gain"
+    (job,) = run_batch([Job(sc)])
+    assert job.adjustments() == padj
+    assert job.agc.gain == pgain
+    assert np.array_equal(job.result(), pb)
+    assert job.clipped.value == int(pc.sum())
+
+
+def test_uneven_batch_each_job_as_alone():
+    jobs_sc = S.c3(seed=2, count=6, epochs_cap=230) + [_strong_iq4(9, spe=30_690, epochs=310)]
+    print("case 7 uneven jobs", [(s.samples_per_epoch, s.format, len(s.channels), s.epochs) for s in jobs_sc], flush=True)
+    jobs = run_batch([Job(s) for s in jobs_sc])
+    for s, j in zip(jobs_sc, jobs):
+        pb, pc, pgain, padj = run_plain(s)
+        assert np.array_equal(j.result(), pb), s.name
+        assert j.clipped.value == int(pc.sum()), s.name
+        assert j.adjustments() == padj and j.agc.gain == pgain, s.name
+
+
+def test_batch_from_a_mid_run_epoch():
+    sc = _strong_iq4(3, epochs=420)
+    print("case adaptive gain from epoch 137", flush=True)
+    rows, masks = sc.window(137, 283)
+    pb, pc, pgain, padj = run_plain(sc, first_epoch=137, epochs=283)
+    (job,) = run_batch([Job(sc, first_epoch=137, epochs=283, rows=rows, masks=masks)])
+    assert np.array_equal(job.result(), pb)
+    assert job.adjustments() == padj and job.agc.gain == pgain
