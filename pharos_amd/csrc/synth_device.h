@@ -14,7 +14,7 @@ constexpr uint32_t STEPS = RESEED / LANES16;  // 32 advances per block
 constexpr uint32_t TILE_BLOCKS = 8;
 constexpr uint32_t THREADS = TILE_BLOCKS * LANES16;  // 128 = two wave64
 constexpr uint32_t TILE = TILE_BLOCKS * RESEED;      // 4096 samples
-constexpr uint32_t MAX_CHUNK = 16;   // channels staged per LDS pass
+constexpr uint32_t MAX_CHUNK = 16;   // channels staged per LDS pass (at most THREADS / 8: one uint4 of a row per thread)
 constexpr uint32_t LEAD_ALLOW = 64;  // entries a leading component may be ahead
 constexpr double RATE_MARGIN = 1.031; // derive() admits 3 % (parameters.zig:46,55)
 constexpr uint32_t VT_BYTES = 256;   // value table: 4 period offsets x 8 states x (re, im)

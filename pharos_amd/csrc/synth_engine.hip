@@ -67,7 +67,7 @@ __global__ __launch_bounds__(THREADS) void synth_window_kernel(const DevEngine *
     A.tile = tile;
     A.epoch = first_epoch + e;
     A.out_epoch = out + (size_t)e * E->bpe;
-    uint32_t clips = synth_tile(A, smem, [&]() { return gain; }, err);
+    uint32_t clips = synth_tile(A, smem, [&]() { return gain; }, []() {}, err);
     if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
 }
 
@@ -86,6 +86,7 @@ struct BatchState {
     DevAdjust *adjust;             // per job, adjust_cap each
     uint32_t adjust_cap;
     uint32_t *err;
+    uint32_t queue_words_offset;   // LDS byte offset of the work-queue words
 };
 
 __device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
@@ -137,23 +138,27 @@ __device__ void agc_account(const BatchState &S, const DevJob &J, uint32_t job, 
 
 __global__ __launch_bounds__(THREADS) void synth_batch_kernel(BatchState S) {
     extern __shared__ __align__(16) uint8_t smem[];
-    __shared__ unsigned long long s_item;
-    __shared__ uint32_t s_seg;
-    __shared__ float s_gain;
-    uint32_t hint = 0;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            unsigned long long item = atomicAdd(S.queue, 1ull);
-            s_item = item;
-            if (item < S.total_items) {
-                hint = find_segment(S, item, hint);
-                s_seg = hint;
-            }
+    // Queue words live after the largest layout of any job (synth_tile's reduction words sit below them).
+    // q[0..1]: the item, q[2]: its segment, q[3]: the gain broadcast.
+    volatile uint32_t *q = reinterpret_cast<volatile uint32_t *>(smem + S.queue_words_offset);
+    uint32_t hint = 0;  // thread 0's segment cursor
+    if (threadIdx.x == 0) {
+        const unsigned long long item = atomicAdd(S.queue, 1ull);
+        q[0] = (uint32_t)item;
+        q[1] = (uint32_t)(item >> 32);
+        if (item < S.total_items) {
+            hint = find_segment(S, item, hint);
+            q[2] = hint;
         }
+    }
+    for (;;) {
         __syncthreads();
-        const unsigned long long item = s_item;
+        const unsigned long long item = (unsigned long long)q[0] | ((unsigned long long)q[1] << 32);
         if (item >= S.total_items) break;
-        const uint32_t s = s_seg;
+        const uint32_t s = q[2];
+        // The next item is taken now and resolved while this one computes (AfterSums below).
+        unsigned long long next_item = 0;
+        if (threadIdx.x == 0) next_item = atomicAdd(S.queue, 1ull);
         const DevSegment seg = S.segs[s];
         const DevJob &J = S.jobs[seg.job];
         const uint64_t local = item - seg.first_item;
@@ -172,12 +177,22 @@ __global__ __launch_bounds__(THREADS) void synth_batch_kernel(BatchState S) {
             if (threadIdx.x == 0) {
                 while (__hip_atomic_load(&S.seg_ready[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
                     __builtin_amdgcn_s_sleep(8);
-                s_gain = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                q[3] = __float_as_uint(__hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
             }
             __syncthreads();
-            return s_gain;
+            return __uint_as_float(q[3]);
         };
-        const uint32_t clips = synth_tile(A, smem, gain_fn, S.err);
+        auto after_sums = [&]() {
+            if (threadIdx.x == 0) {
+                q[0] = (uint32_t)next_item;
+                q[1] = (uint32_t)(next_item >> 32);
+                if (next_item < S.total_items) {
+                    hint = find_segment(S, next_item, hint);
+                    q[2] = hint;
+                }
+            }
+        };
+        const uint32_t clips = synth_tile(A, smem, gain_fn, after_sums, S.err);
         if (threadIdx.x == 0) {
             atomicAdd(&S.seg_clip[s], (unsigned long long)clips);
             __threadfence();
@@ -191,7 +206,6 @@ __global__ __launch_bounds__(THREADS) void synth_batch_kernel(BatchState S) {
                 }
             }
         }
-        __syncthreads();
     }
 }
 
@@ -357,13 +371,15 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
     e->view.w_slot = (std::max<uint32_t>(w_slot_max, 16) + 15) / 16 * 16;
     // Channels per LDS pass: as many as fit a 48 KiB budget next to the tile's output units.
     // Channels per LDS pass: as many as fit a 32 KiB budget; the tile's sums reuse the same bytes afterwards.
-    const uint32_t budget = SUM_BYTES;
+    const uint32_t budget = SUM_BYTES - ROWS_BYTES;
     const uint32_t per_ch = (uint32_t)sizeof(ChanRec) + VT_BYTES + e->view.w_slot;
     uint32_t chunk = budget / per_ch;
     chunk = std::max<uint32_t>(1, std::min<uint32_t>(chunk, MAX_CHUNK));
     e->view.chunk = chunk;
     e->view.out_bytes = 0;
-    e->view.lds_bytes = std::max<uint32_t>(chunk * per_ch, SUM_BYTES) + 64;
+    // Layout (synth_kernel.h lds_layout): rows, records, value tables, staging; the sums alias them; 64 bytes of
+    // reduction words follow.
+    e->view.lds_bytes = std::max<uint32_t>(ROWS_BYTES + chunk * per_ch, SUM_BYTES) + 64;
     if (e->view.lds_bytes > 160 * 1024) {
         set_error("a channel's code tables need %u bytes of LDS per workgroup", e->view.lds_bytes);
         engine_free(e);
@@ -499,6 +515,8 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
         lds = std::max(lds, J.engine->view.lds_bytes);
     }
     HIP_TRY(hipSetDevice(device), ST_INTERNAL);
+    const uint32_t queue_words_offset = (lds + 15u) & ~15u;
+    lds = queue_words_offset + 16;
     pharos_synth_batch_handle *h = new (std::nothrow) pharos_synth_batch_handle();
     if (!h) { set_error("out of memory"); return ST_NOMEM; }
     h->device = device;
@@ -624,6 +642,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.adjust = (DevAdjust *)(b8 + p_adj.off);
     S.adjust_cap = adjust_cap;
     S.err = (uint32_t *)(b8 + p_err.off);
+    S.queue_words_offset = queue_words_offset;
     h->d_agc = S.agc;
     h->d_adjust = S.adjust;
     h->d_seg_clip = S.seg_clip;
@@ -636,6 +655,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
         per_cu = (uint32_t)occ;
     }
     uint64_t grid = (uint64_t)cu_count(device) * per_cu;
+    (void)0;
     if (grid > items) grid = items;
     if (grid == 0) grid = 1;
     hipLaunchKernelGGL(synth_batch_kernel, dim3((uint32_t)grid), dim3(THREADS), lds, h->stream, S);

@@ -104,10 +104,63 @@ __device__ __forceinline__ void noise_sample(uint64_t seed, uint64_t a, float &r
     im = r * s;
 }
 
+// The four samples of noise group g (noise.zig:31-43), computed together as the reference does: 4 counters for the
+// radius and angle words, 2 more for the low words when a radius word is small.  (g * 6 + k) * golden is formed
+// as g * (6 golden) + k golden, the same value mod 2^64.
+__device__ __forceinline__ void noise_group4(uint64_t seed, uint64_t g, float (&re)[4], float (&im)[4]) {
+    constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
+    const uint64_t base = seed + g * (6ull * GOLDEN);
+    auto mix = [&](uint64_t k) {
+        uint64_t z = base + k * GOLDEN;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    const uint64_t h0 = mix(0), h1 = mix(1), a0 = mix(4), a1 = mix(5);
+    const uint32_t hi[4] = {(uint32_t)h0, (uint32_t)(h0 >> 32), (uint32_t)h1, (uint32_t)(h1 >> 32)};
+    const uint32_t ang[4] = {(uint32_t)a0, (uint32_t)(a0 >> 32), (uint32_t)a1, (uint32_t)(a1 >> 32)};
+    float v[4];
+#pragma unroll
+    for (int l = 0; l < 4; l++) v[l] = (float)hi[l];
+    if (hi[0] < (1u << 25) || hi[1] < (1u << 25) || hi[2] < (1u << 25) || hi[3] < (1u << 25)) {
+        const uint64_t l0 = mix(2), l1 = mix(3);
+        const uint32_t lo[4] = {(uint32_t)l0, (uint32_t)(l0 >> 32), (uint32_t)l1, (uint32_t)(l1 >> 32)};
+#pragma unroll
+        for (int l = 0; l < 4; l++) v[l] = fmaf((float)(lo[l] | 1u), 0x1p-32f, (float)hi[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        float n2 = fmaf(log_poly(v[l]), -2.0f, 0x1.62e43p+5f);
+        n2 = n2 > 0.0f ? n2 : 0.0f;
+        const float r = sqrtf(n2);
+        float c, s;
+        sincos_turn(ang[l], c, s);
+        re[l] = r * c;
+        im[l] = r * s;
+    }
+}
+
+// ------------------------------------------------------------------ raw LDS addressing
+// 32-bit LDS byte addresses, dereferenced through address-space-3 pointers so a per-sample address needs no base add.
+typedef __attribute__((address_space(3))) uint8_t lds_byte;
+typedef __attribute__((address_space(3))) float lds_float;
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
+__device__ __forceinline__ uint32_t lds_address(const void *p) {
+    return (uint32_t)(uintptr_t)(const lds_byte *)p;
+}
+__device__ __forceinline__ uint32_t lds_load_u8(uint32_t a) { return *(const lds_byte *)(uintptr_t)a; }
+__device__ __forceinline__ float lds_load_f32(uint32_t a) { return *(const lds_float *)(uintptr_t)a; }
+__device__ __forceinline__ float2 lds_load_f2(uint32_t a) {
+    const unsigned long long v = *(const lds_u64 *)(uintptr_t)a;
+    return make_float2(__uint_as_float((uint32_t)v), __uint_as_float((uint32_t)(v >> 32)));
+}
+
 // ------------------------------------------------------------------ LDS layout of a work item
 // The channel region (records, value tables, staged entries) and, after the channel loop, the tile's sums
 // (TILE f32 I then TILE f32 Q) share the same bytes.
 struct Lds {
+    pharos_synth_row *rows;  // the chunk's rows (MAX_CHUNK)
     ChanRec *rec;
     uint8_t *vt;      // chunk x VT_BYTES
     uint8_t *stage;   // chunk x w_slot
@@ -117,11 +170,14 @@ struct Lds {
 };
 
 constexpr uint32_t SUM_BYTES = TILE * 8;
+constexpr uint32_t ROWS_BYTES = MAX_CHUNK * (uint32_t)sizeof(pharos_synth_row);
 
 __device__ __forceinline__ Lds lds_layout(uint8_t *smem, const DevEngine &E) {
     Lds L;
-    L.rec = reinterpret_cast<ChanRec *>(smem);
-    uint32_t off = E.chunk * (uint32_t)sizeof(ChanRec);
+    L.rows = reinterpret_cast<pharos_synth_row *>(smem);
+    uint32_t off = ROWS_BYTES;
+    L.rec = reinterpret_cast<ChanRec *>(smem + off);
+    off += E.chunk * (uint32_t)sizeof(ChanRec);
     L.vt = smem + off;
     L.vt_base = off;
     off += E.chunk * VT_BYTES;
@@ -141,9 +197,10 @@ __device__ __forceinline__ uint8_t mask_at(const ChanRec &R, const uint8_t *mask
     return at < mask_len ? masks[at] : 0;
 }
 
-// Phase 1: one thread per channel of the chunk derives the channel's per-tile constants.
-__device__ void chan_setup(ChanRec &R, const pharos_synth_row &row, const DevEngine &E, uint32_t n0, uint32_t tile_len,
-                           uint64_t mask_len, uint32_t *err) {
+// Setup stage A, one thread per channel of the chunk: the channel's code edge, staged window and shape for this
+// tile, from its row (already in LDS).  No rotor work here.
+__device__ void chan_core(ChanRec &R, const pharos_synth_row &row, const DevEngine &E, uint32_t n0, uint32_t tile_len,
+                          uint64_t mask_len, uint32_t *err) {
     R.active = 0;
     if (row.table == PHAROS_SYNTH_INACTIVE) return;
     if (row.table >= E.ntables) { atomicOr(err, ERR_TABLE); return; }
@@ -153,35 +210,35 @@ __device__ void chan_setup(ChanRec &R, const pharos_synth_row &row, const DevEng
         return;
     }
     // chip.Edge.of (chip.zig:17-24)
-    double fl = floor(row.entry_phase0);
-    uint64_t first = (uint64_t)fl;
-    uint64_t phase_q = frac_q(row.entry_phase0 - fl);
-    uint64_t eps_q = (uint64_t)round(row.entries_per_sample * 4294967296.0);
+    const double fl = floor(row.entry_phase0);
+    const uint64_t first = (uint64_t)fl;
+    const uint64_t phase_q = frac_q(row.entry_phase0 - fl);
+    const uint64_t eps_q = (uint64_t)round(row.entries_per_sample * 4294967296.0);
     if (eps_q == 0) { atomicOr(err, ERR_TABLE); return; }
-    uint64_t P0 = phase_q + (uint64_t)n0 * eps_q;
-    uint64_t Pl = phase_q + (uint64_t)(n0 + tile_len - 1) * eps_q;
-    uint64_t e_lo = first + (P0 >> 32);
-    uint32_t shape = T.tdm_periods ? SHAPE_TDM : (row.lead_bits ? SHAPE_LEAD : SHAPE_PLAIN);
+    const uint64_t P0 = phase_q + (uint64_t)n0 * eps_q;
+    const uint64_t Pl = phase_q + (uint64_t)(n0 + tile_len - 1) * eps_q;
+    const uint64_t e_lo = first + (P0 >> 32);
+    const uint32_t per = T.per;
+    const uint32_t shape = T.tdm_periods ? SHAPE_TDM : (row.lead_bits ? SHAPE_LEAD : SHAPE_PLAIN);
     uint32_t lead_whole = 0, lead_q = 0;
     if (shape == SHAPE_LEAD) {
         if (!(row.lead_entries >= 0.0) || !(row.lead_entries < (double)LEAD_ALLOW)) { atomicOr(err, ERR_STAGE); return; }
-        double lw = floor(row.lead_entries);
+        const double lw = floor(row.lead_entries);
         lead_whole = (uint32_t)lw;
         lead_q = frac_q(row.lead_entries - lw);  // chip.zig:162-165
     }
-    uint64_t W = (Pl >> 32) - (P0 >> 32) + 2 + lead_whole;
-    uint64_t k_lo = e_lo / T.per, e0 = e_lo % T.per;
+    const uint64_t W = (Pl >> 32) - (P0 >> 32) + 2 + lead_whole;
+    const uint64_t k_lo = e_lo / per, e0 = e_lo % per;
     if (W > T.w_max || e0 + W > T.ext_len) { atomicOr(err, ERR_STAGE); return; }
     // The last entry a live sample reads (the lead adds at most one carry), and the period it falls in: the
     // reference's mask slice covers masksNeeded periods (parameters.zig:25-29), so a row reading past the host's
     // mask bytes is a host error.
     const uint64_t last_rel = (Pl >> 32) - (P0 >> 32) + lead_whole + (shape == SHAPE_LEAD ? 1u : 0u);
-    const uint64_t w_used = (e0 + last_rel) / T.per;
+    const uint64_t w_used = (e0 + last_rel) / per;
     if (k_lo + w_used >= row.mask_count || (uint64_t)row.mask_offset + k_lo + w_used >= mask_len) {
         atomicOr(err, ERR_MASKS);
         return;
     }
-    R.active = 1;
     R.shape = shape;
     R.axis = T.axis;
     R.table = row.table;
@@ -197,23 +254,28 @@ __device__ void chan_setup(ChanRec &R, const pharos_synth_row &row, const DevEng
     R.mask_count = row.mask_count;
     R.phase0 = row.carrier_phase0;
     R.step = row.carrier_step;
-    // Rotor.init (rotor.zig:21-35): the lane powers by the same f64 recursion, and the f32 16-sample step.
-    double w1c, w1s;
-    sincos64(row.carrier_step, E.turn, w1c, w1s);
+    R.active = 1;
+}
+
+// Setup stage B, one thread per channel: Rotor.init (rotor.zig:21-35), the lane powers by the same f64 recursion
+// and the f32 16-sample step.
+__device__ void chan_rotor(ChanRec &R, const double *turn) {
+    if (!R.active) return;
+    double w1c, w1s, wc, ws;
+    sincos64(R.step, turn, w1c, w1s);
+    sincos64(16.0 * R.step, turn, wc, ws);
     double ac = 1.0, as = 0.0;
     R.pw_c[0] = 1.0;
     R.pw_s[0] = 0.0;
 #pragma unroll
     for (uint32_t l = 1; l < LANES16; l++) {
-        double nc = ac * w1c - as * w1s;
-        double ns = ac * w1s + as * w1c;
+        const double nc = ac * w1c - as * w1s;
+        const double ns = ac * w1s + as * w1c;
         ac = nc;
         as = ns;
         R.pw_c[l] = ac;
         R.pw_s[l] = as;
     }
-    double wc, ws;
-    sincos64(16.0 * row.carrier_step, E.turn, wc, ws);
     R.w_re = (float)wc;
     R.w_im = (float)ws;
 }
@@ -221,39 +283,39 @@ __device__ void chan_setup(ChanRec &R, const pharos_synth_row &row, const DevEng
 // The per-sample loop of one channel for one thread: 32 samples at stride 16 of one rotor block.
 // pos: high word = LDS address of the staged entry byte, low word = Q32.32 fraction of the code position.
 template <uint32_t AXIS, bool LEAD>
-__device__ __forceinline__ void channel_samples(const uint8_t *smem, float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos,
-                                                uint64_t step, uint32_t vt_off, float cre, float cim, float wre,
-                                                float wim, uint64_t lead_add, uint32_t lead_bits) {
+__device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos, uint64_t step,
+                                                uint32_t vt_addr, float cre, float cim, float wre, float wim,
+                                                uint64_t lead_add, uint32_t lead_bits) {
     const float nwim = -wim;
 #pragma unroll
     for (uint32_t k = 0; k < STEPS; k++) {
-        uint32_t a = (uint32_t)(pos >> 32);
-        uint32_t voff;
+        const uint32_t a = (uint32_t)(pos >> 32);
+        uint32_t va;
         if (LEAD) {
             // chip.zig:143-159 / kernel.zig:233-234: the leading bits come from the entry the lead reaches.
-            uint32_t al = (uint32_t)((pos + lead_add) >> 32);
-            uint32_t bm = smem[a], bl = smem[al];
-            uint32_t idx = (bl & lead_bits) | (bm & ~lead_bits);
-            voff = vt_off + idx * 8u;
+            const uint32_t al = (uint32_t)((pos + lead_add) >> 32);
+            const uint32_t bm = lds_load_u8(a), bl = lds_load_u8(al);
+            const uint32_t idx = (bl & lead_bits) | (bm & ~lead_bits);
+            va = vt_addr + idx * 8u;
         } else {
-            voff = vt_off + smem[a];
+            va = vt_addr + lds_load_u8(a);
         }
         // kernel.zig:241-258
         if (AXIS == AXIS_BOTH) {
-            float2 v = *reinterpret_cast<const float2 *>(smem + voff);
+            const float2 v = lds_load_f2(va);
             ar[k] = fmaf(v.x, cre, fmaf(-v.y, cim, ar[k]));
             ai[k] = fmaf(v.x, cim, fmaf(v.y, cre, ai[k]));
         } else if (AXIS == AXIS_RE) {
-            float v = *reinterpret_cast<const float *>(smem + voff);
+            const float v = lds_load_f32(va);
             ar[k] = fmaf(v, cre, ar[k]);
             ai[k] = fmaf(v, cim, ai[k]);
         } else {
-            float v = *reinterpret_cast<const float *>(smem + voff + 4);
+            const float v = lds_load_f32(va + 4);
             ar[k] = fmaf(-v, cim, ar[k]);
             ai[k] = fmaf(v, cre, ai[k]);
         }
         // Rotor.advance (rotor.zig:52-58)
-        float t = fmaf(cre, wre, cim * nwim);
+        const float t = fmaf(cre, wre, cim * nwim);
         cim = fmaf(cre, wim, cim * wre);
         cre = t;
         pos += step;
@@ -278,10 +340,22 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
     const float lo = F == PHAROS_SYNTH_IQ8 ? -128.0f : -32768.0f, hi = F == PHAROS_SYNTH_IQ8 ? 127.0f : 32767.0f;
     const float thr = 1.1f / gain, clip_thr = 5.0f * thr;
     uint32_t clip = 0;
+    // Groups of 4 samples line up with the noise groups when the tile starts on a multiple of 4 absolute samples.
+    const bool grouped = (abs0 & 3u) == 0;
     for (uint32_t g0 = threadIdx.x * G; g0 < valid; g0 += THREADS * G) {
         uint32_t w[NW];
 #pragma unroll
         for (uint32_t q = 0; q < NW; q++) w[q] = 0;
+        float nre[G], nim[G];
+        if (noisy && grouped) {
+#pragma unroll
+            for (uint32_t h = 0; h < G; h += 4) {
+                float r4[4], i4[4];
+                noise_group4(E.seed, (abs0 + g0 + h) >> 2, r4, i4);
+#pragma unroll
+                for (uint32_t l = 0; l < 4; l++) { nre[h + l] = r4[l]; nim[h + l] = i4[l]; }
+            }
+        }
 #pragma unroll
         for (uint32_t j = 0; j < G; j++) {
             const uint32_t i = g0 + j;
@@ -289,7 +363,8 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
                 float a = L.sum_re[i], c = L.sum_im[i];
                 if (noisy) {
                     float nr, ni;
-                    noise_sample(E.seed, abs0 + i, nr, ni);
+                    if (grouped) { nr = nre[j]; ni = nim[j]; }
+                    else noise_sample(E.seed, abs0 + i, nr, ni);
                     a = a + nr;
                     c = c + ni;
                 }
@@ -348,10 +423,11 @@ struct TileArgs {
 };
 
 
-// One work item.  GainFn is called by every thread once the sums are complete (the persistent kernel waits there
-// for the AGC of the previous window).  Returns the clipped components of the tile in thread 0.
-template <class GainFn>
-__device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, uint32_t *err) {
+// One work item.  AfterSums is called by every thread when the last chunk's sums are done (the persistent kernel
+// fetches its next item there); GainFn by every thread before quantising (the persistent kernel waits there for the
+// AGC of the previous window).  Returns the clipped components of the tile.
+template <class GainFn, class AfterSums>
+__device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, AfterSums after_sums, uint32_t *err) {
     const DevEngine &E = *A.E;
     const uint32_t tid = threadIdx.x;
     const uint32_t b = tid / LANES16, l = tid % LANES16;
@@ -361,45 +437,57 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
     const uint32_t nb = n0 + b * RESEED;
     const bool block_live = nb < E.spe;
     Lds L = lds_layout(smem, E);
+    const uint32_t smem_lds = lds_address(smem);
 
     float ar[STEPS], ai[STEPS];
 #pragma unroll
     for (uint32_t k = 0; k < STEPS; k++) { ar[k] = 0.0f; ai[k] = 0.0f; }
 
+    // Rows of the first chunk into LDS; later chunks' rows are fetched into registers during the previous compute.
+    const uint32_t row_words = (uint32_t)sizeof(pharos_synth_row) / 16u;  // uint4 per row
+    {
+        const uint32_t nc = min(E.chunk, A.channels);
+        if (tid < nc * row_words)
+            reinterpret_cast<uint4 *>(L.rows)[tid] = reinterpret_cast<const uint4 *>(A.rows)[tid];
+    }
+    __syncthreads();
     for (uint32_t c0 = 0; c0 < A.channels; c0 += E.chunk) {
         const uint32_t nc = min(E.chunk, A.channels - c0);
-        if (tid < nc) chan_setup(L.rec[tid], A.rows[c0 + tid], E, n0, tile_len, A.mask_len, err);
+        // Stage A: per-channel edge, window and shape (LDS and table descriptors only).
+        if (tid < nc) chan_core(L.rec[tid], L.rows[tid], E, n0, tile_len, A.mask_len, err);
         __syncthreads();
-        // Phase 2a: block seeds, sincos64 of each block's start (Rotor.seed, rotor.zig:37-38).
+        // Stage B: one round of independent global loads.  Lane powers (serial, one thread per channel) ...
+        if (tid < nc) chan_rotor(L.rec[tid], E.turn);
+        // ... block seeds, sincos64 of each block's start (Rotor.seed, rotor.zig:37-38) ...
         for (uint32_t i = tid; i < nc * TILE_BLOCKS; i += THREADS) {
             ChanRec &R = L.rec[i / TILE_BLOCKS];
-            uint32_t bb = i % TILE_BLOCKS;
+            const uint32_t bb = i % TILE_BLOCKS;
             if (!R.active) continue;
             double c, s;
             sincos64(R.phase0 + (double)(n0 + bb * RESEED) * R.step, E.turn, c, s);
             R.c0_c[bb] = c;
             R.c0_s[bb] = s;
         }
-        // Phase 2b: value tables, the symbol masks folded in per period offset (kernel.zig:104-110,183,202).
+        // ... value tables, the symbol masks folded in per period offset (kernel.zig:104-110,183,202) ...
         for (uint32_t i = tid; i < nc * 32; i += THREADS) {
             const uint32_t ch = i / 32, w = (i / 8) % 4, s = i % 8;
             const ChanRec &R = L.rec[ch];
             if (!R.active) continue;
             const DevTable &T = E.tables[R.table];
-            const float amp = A.rows[c0 + ch].amp;
+            const float amp = L.rows[ch].amp;
             uint32_t idx;
             if (R.shape == SHAPE_LEAD) {
                 if (w != 0) continue;
                 idx = s;
             } else {
-                uint32_t m = mask_at(R, A.masks, A.mask_len, R.k_lo + w);
+                const uint32_t m = mask_at(R, A.masks, A.mask_len, R.k_lo + w);
                 idx = R.shape == SHAPE_TDM ? (s < 2 ? s ^ (m & 1u) : s) : (s ^ m);
             }
             float2 v = make_float2(0.0f, 0.0f);
             if (idx < T.count) v = make_float2((float)((double)amp * T.vre[idx]), (float)((double)amp * T.vim[idx]));
             *reinterpret_cast<float2 *>(L.vt + ch * VT_BYTES + (w * 8 + s) * 8) = v;
         }
-        // Phase 2c: stage each channel's entries of this tile from its extended table.
+        // ... and each channel's entries of this tile, staged from its extended table.
         for (uint32_t ch = 0; ch < nc; ch++) {
             const ChanRec &R = L.rec[ch];
             if (!R.active) continue;
@@ -415,17 +503,17 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
 #pragma unroll
                     for (uint32_t q = 0; q < 4; q++) {
                         uint32_t x = (v >> (8 * q)) & 0xffu;
-                        uint32_t w = x >> 6;
+                        const uint32_t w = x >> 6;
                         if (R.shape == SHAPE_LEAD) {
                             x = ((x >> 3) & 7u) ^ mask_at(R, A.masks, A.mask_len, R.k_lo + w);
                         } else {
                             // TDM (kernel.zig:197-203): odd entries carry 2 | the pilot chip of their period.
-                            int32_t j = (int32_t)(4 * d + q) - (int32_t)head;
-                            uint32_t e = R.e0 + (uint32_t)(j < 0 ? 0 : j) - w * T.per;
+                            const int32_t j = (int32_t)(4 * d + q) - (int32_t)head;
+                            const uint32_t e = R.e0 + (uint32_t)(j < 0 ? 0 : j) - w * T.per;
                             if (j >= 0 && (e & 1u)) {
-                                uint64_t period = A.rows[c0 + ch].period0 + R.k_lo + w;
-                                uint32_t pi = (uint32_t)(period % T.tdm_periods) * T.tdm_half + e / 2;
-                                uint32_t chip = pi < T.pilot_len ? T.pilot[pi] : 0u;
+                                const uint64_t period = L.rows[ch].period0 + R.k_lo + w;
+                                const uint32_t pi = (uint32_t)(period % T.tdm_periods) * T.tdm_half + e / 2;
+                                const uint32_t chip = pi < T.pilot_len ? T.pilot[pi] : 0u;
                                 x = 8u * (2u | chip) + 64u * w;
                             }
                         }
@@ -437,7 +525,13 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
             }
         }
         __syncthreads();
-        // Phase 3: the channel sums.
+        // The next chunk's rows travel from global memory while this chunk computes.
+        const uint32_t c1 = c0 + E.chunk;
+        const uint32_t nc_next = c1 < A.channels ? min(E.chunk, A.channels - c1) : 0u;
+        uint4 next_row = make_uint4(0, 0, 0, 0);
+        const bool fetch = tid < nc_next * row_words;
+        if (fetch) next_row = reinterpret_cast<const uint4 *>(A.rows + c1)[tid];
+        // The channel sums.
         if (block_live) {
             for (uint32_t ch = 0; ch < nc; ch++) {
                 const ChanRec &R = L.rec[ch];
@@ -447,24 +541,28 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
                 const float cim = (float)(cc * ps + cs * pc);
                 const uint64_t Pn = R.P0 + (uint64_t)(b * RESEED + l) * R.eps_q;
                 const uint32_t rel = (uint32_t)(Pn >> 32) - (uint32_t)(R.P0 >> 32);
-                const uint32_t addr = L.stage_base + ch * E.w_slot + (R.e0 & 3u) + rel;
+                const uint32_t addr = smem_lds + L.stage_base + ch * E.w_slot + (R.e0 & 3u) + rel;
                 const uint64_t pos = ((uint64_t)addr << 32) | (uint32_t)Pn;
                 const uint64_t step = R.eps_q * LANES16;
-                const uint32_t vt_off = L.vt_base + ch * VT_BYTES;
+                const uint32_t vt_off = smem_lds + L.vt_base + ch * VT_BYTES;
                 if (R.shape == SHAPE_LEAD) {
                     const uint64_t lead_add = ((uint64_t)R.lead_whole << 32) + R.lead_q;
-                    if (R.axis == AXIS_BOTH) channel_samples<AXIS_BOTH, true>(smem, ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
-                    else if (R.axis == AXIS_RE) channel_samples<AXIS_RE, true>(smem, ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
-                    else channel_samples<AXIS_IM, true>(smem, ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
+                    if (R.axis == AXIS_BOTH) channel_samples<AXIS_BOTH, true>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
+                    else if (R.axis == AXIS_RE) channel_samples<AXIS_RE, true>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
+                    else channel_samples<AXIS_IM, true>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
                 } else {
-                    if (R.axis == AXIS_BOTH) channel_samples<AXIS_BOTH, false>(smem, ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
-                    else if (R.axis == AXIS_RE) channel_samples<AXIS_RE, false>(smem, ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
-                    else channel_samples<AXIS_IM, false>(smem, ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
+                    if (R.axis == AXIS_BOTH) channel_samples<AXIS_BOTH, false>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
+                    else if (R.axis == AXIS_RE) channel_samples<AXIS_RE, false>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
+                    else channel_samples<AXIS_IM, false>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
                 }
             }
         }
         __syncthreads();
+        if (fetch) reinterpret_cast<uint4 *>(L.rows)[tid] = next_row;
+        if (c1 >= A.channels) after_sums();
+        __syncthreads();
     }
+    if (A.channels == 0) after_sums();
 
     // The sums leave the rotor-lane layout for LDS; the epilogue then walks contiguous samples.
     if (block_live) {
