@@ -17,13 +17,15 @@ constexpr uint32_t TILE = TILE_BLOCKS * RESEED;      // 4096 samples
 constexpr uint32_t MAX_CHUNK = 16;   // channels staged per LDS pass (at most THREADS / 8: one uint4 of a row per thread)
 constexpr uint32_t LEAD_ALLOW = 64;  // entries a leading component may be ahead
 constexpr double RATE_MARGIN = 1.031; // derive() admits 3 % (parameters.zig:46,55)
-constexpr uint32_t VT_BYTES = 256;   // value table: 4 period offsets x 8 states x (re, im)
+constexpr uint32_t VT_BYTES = 128;   // value table: 16 x (re, im) f32, entries from the table's count on zero
 
 enum Shape : uint32_t { SHAPE_PLAIN = 0, SHAPE_TDM = 1, SHAPE_LEAD = 2 };
 enum Axis : uint32_t { AXIS_BOTH = 0, AXIS_RE = 1, AXIS_IM = 2 };
 
 // A code table on the device.  ext[i] = 8 * states[i % per] + 64 * (i / per) for i < ext_len: one period plus
-// the entries a tile can run past its end, each byte naming its state and its period offset (0..3).
+// the entries a tile can run past its end, each byte naming its state and its period offset (0..3).  Staging a
+// tile turns each byte into 8 x its value index (the state with its period's symbol mask applied), the byte offset
+// of the value in the channel's value table.
 struct DevTable {
     const uint8_t *ext;
     const uint8_t *pilot;      // TDM pilot chips, else null
@@ -53,14 +55,15 @@ struct DevEngine {
 struct ChanRec {
     double pw_c[LANES16], pw_s[LANES16];  // w1^l, the rotor's f64 lane powers (rotor.zig:25-30)
     double c0_c[TILE_BLOCKS], c0_s[TILE_BLOCKS];  // sincos64 of each block's start phase (rotor.zig:37-38)
-    double phase0, step;
+    const uint32_t *src;       // the staged window's first dword in the extended table (4-byte aligned)
     uint64_t P0;               // Q32.32 code position of the tile's first sample, less `first`
     uint64_t eps_q;
     float w_re, w_im;          // f32 sincos64(16 step) (rotor.zig:31-33)
     uint32_t active, shape, axis, table;
     uint32_t e0, k_lo, W, lead_q;
-    uint32_t lead_whole, lead_bits, mask_offset, mask_count;
-    uint32_t tdm_pad[2];
+    uint32_t lead_whole, lead_bits, ndw, m4;  // staged dwords; symbol masks of period offsets 0..3, a byte each
+    uint32_t wide_mask;        // some mask byte exceeds 7 (a value index past any table: staged as the zero entry)
+    uint32_t pad[3];
 };
 static_assert(sizeof(ChanRec) % 16 == 0, "ChanRec keeps 16-byte alignment");
 

@@ -50,7 +50,10 @@ __global__ void clear_words(uint32_t *p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-__global__ __launch_bounds__(THREADS) void synth_window_kernel(const DevEngine *__restrict__ E,
+// One kernel per output format: each carries only its own quantiser, which keeps the code within the instruction
+// cache.
+template <uint32_t F>
+__global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngine *__restrict__ E,
                                                                const pharos_synth_row *__restrict__ rows,
                                                                uint32_t channels, const uint8_t *__restrict__ masks,
                                                                uint64_t mask_len, uint64_t first_epoch, float gain,
@@ -67,7 +70,7 @@ __global__ __launch_bounds__(THREADS) void synth_window_kernel(const DevEngine *
     A.tile = tile;
     A.epoch = first_epoch + e;
     A.out_epoch = out + (size_t)e * E->bpe;
-    uint32_t clips = synth_tile(A, smem, [&]() { return gain; }, []() {}, err);
+    uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain; }, []() {}, err);
     if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
 }
 
@@ -76,7 +79,7 @@ struct BatchState {
     const DevSegment *segs;
     const uint32_t *seg_next;      // the same job's next segment, or ~0
     uint32_t nsegs;
-    uint64_t total_items;
+    uint64_t total_items;          // end of this launch's items (they start where its queue starts)
     unsigned long long *queue;     // next work item
     uint32_t *tiles_done;          // per segment
     unsigned long long *seg_clip;  // per segment
@@ -90,27 +93,29 @@ struct BatchState {
 };
 
 __device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
-    if (hint < S.nsegs && S.segs[hint].first_item <= item &&
-        (hint + 1 == S.nsegs || S.segs[hint + 1].first_item > item))
+    const auto segs = gmem(S.segs);
+    if (hint < S.nsegs && segs[hint].first_item <= item &&
+        (hint + 1 == S.nsegs || segs[hint + 1].first_item > item))
         return hint;
-    if (hint + 1 < S.nsegs && S.segs[hint + 1].first_item <= item &&
-        (hint + 2 == S.nsegs || S.segs[hint + 2].first_item > item))
+    if (hint + 1 < S.nsegs && segs[hint + 1].first_item <= item &&
+        (hint + 2 == S.nsegs || segs[hint + 2].first_item > item))
         return hint + 1;
     uint32_t lo = 0, hi = S.nsegs;  // first_item[lo] <= item < first_item[hi]
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) / 2;
-        if (S.segs[mid].first_item <= item) lo = mid; else hi = mid;
+        if (segs[mid].first_item <= item) lo = mid; else hi = mid;
     }
     return lo;
 }
 
 // One AGC step after a window (gain.zig:19-35, accounted as engine/run.zig:240-246 does per window).
-__device__ void agc_account(const BatchState &S, const DevJob &J, uint32_t job, const DevSegment &seg, uint32_t s) {
+__device__ void agc_account(const BatchState &S, uint64_t first_epoch, uint32_t spe, uint32_t job, const DevSegment &seg,
+                            uint32_t s) {
     DevAgc &G = S.agc[job];
     unsigned long long clipped = __hip_atomic_load(&S.seg_clip[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     G.clipped += clipped;
-    G.components += (unsigned long long)seg.epochs * J.engine->spe * 2ull;
-    const unsigned long long epoch_after = J.first_epoch + seg.epoch_off + seg.epochs;
+    G.components += (unsigned long long)seg.epochs * spe * 2ull;
+    const unsigned long long epoch_after = first_epoch + seg.epoch_off + seg.epochs;
     if (epoch_after % 100ull == 0) {
         const double rate = (double)G.clipped / (double)G.components;
         G.clipped = 0;
@@ -136,7 +141,8 @@ __device__ void agc_account(const BatchState &S, const DevJob &J, uint32_t job, 
     }
 }
 
-__global__ __launch_bounds__(THREADS) void synth_batch_kernel(BatchState S) {
+template <uint32_t F>
+__global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
     extern __shared__ __align__(16) uint8_t smem[];
     // Queue words live after the largest layout of any job (synth_tile's reduction words sit below them).
     // q[0..1]: the item, q[2]: its segment, q[3]: the gain broadcast.
@@ -159,8 +165,16 @@ __global__ __launch_bounds__(THREADS) void synth_batch_kernel(BatchState S) {
         // The next item is taken now and resolved while this one computes (AfterSums below).
         unsigned long long next_item = 0;
         if (threadIdx.x == 0) next_item = atomicAdd(S.queue, 1ull);
-        const DevSegment seg = S.segs[s];
-        const DevJob &J = S.jobs[seg.job];
+        DevSegment seg;
+        {
+            const auto &g = gmem(S.segs)[s];
+            seg.first_item = g.first_item;
+            seg.job = g.job;
+            seg.epoch_off = g.epoch_off;
+            seg.epochs = g.epochs;
+            seg.tiles = g.tiles;
+        }
+        const auto &J = gmem(S.jobs)[seg.job];
         const uint64_t local = item - seg.first_item;
         const uint32_t e_off = seg.epoch_off + (uint32_t)(local / seg.tiles);
         TileArgs A;
@@ -192,14 +206,14 @@ __global__ __launch_bounds__(THREADS) void synth_batch_kernel(BatchState S) {
                 }
             }
         };
-        const uint32_t clips = synth_tile(A, smem, gain_fn, after_sums, S.err);
+        const uint32_t clips = synth_tile<F>(A, smem, gain_fn, after_sums, S.err);
         if (threadIdx.x == 0) {
             atomicAdd(&S.seg_clip[s], (unsigned long long)clips);
             __threadfence();
             const uint32_t done = atomicAdd(&S.tiles_done[s], 1u) + 1u;
             if (done == seg.epochs * seg.tiles) {
                 __threadfence();
-                if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) agc_account(S, J, seg.job, seg, s);
+                if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) agc_account(S, J.first_epoch, gmem(J.engine)->spe, seg.job, seg, s);
                 else {
                     const uint32_t nxt = S.seg_next[s];
                     if (nxt != 0xffffffffu) __hip_atomic_store(&S.seg_ready[nxt], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -368,8 +382,8 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
     e->view.format = c.format;
     e->view.noise = c.noise;
     e->view.tiles = (spe + TILE - 1) / TILE;
-    e->view.w_slot = (std::max<uint32_t>(w_slot_max, 16) + 15) / 16 * 16;
-    // Channels per LDS pass: as many as fit a 48 KiB budget next to the tile's output units.
+    // Slots 32 bytes past a multiple of 64: the eight channels a wave stages together start on distinct banks.
+    e->view.w_slot = (std::max<uint32_t>(w_slot_max, 16) + 31) / 64 * 64 + 32;
     // Channels per LDS pass: as many as fit a 32 KiB budget; the tile's sums reuse the same bytes afterwards.
     const uint32_t budget = SUM_BYTES - ROWS_BYTES;
     const uint32_t per_ch = (uint32_t)sizeof(ChanRec) + VT_BYTES + e->view.w_slot;
@@ -436,9 +450,24 @@ extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch,
     }
     for (uint32_t done = 0; done < epochs;) {
         const uint32_t n = std::min<uint32_t>(epochs - done, 65535u);
-        hipLaunchKernelGGL(synth_window_kernel, dim3(tiles, n), dim3(THREADS), e->view.lds_bytes, stream, e->d_engine,
-                           rows + (size_t)done * channels, channels, masks, mask_len, first_epoch + done, gain,
-                           out + (size_t)done * e->view.bpe, e->d_clip + (size_t)done * tiles, e->d_err);
+        const dim3 grid(tiles, n);
+        const pharos_synth_row *r = rows + (size_t)done * channels;
+        uint8_t *o = out + (size_t)done * e->view.bpe;
+        uint32_t *cp = e->d_clip + (size_t)done * tiles;
+        const uint32_t lds = e->view.lds_bytes;
+        switch (e->view.format) {
+#define PHAROS_WINDOW(FMT)                                                                                         \
+        case FMT:                                                                                                \
+            hipLaunchKernelGGL(synth_window_kernel<FMT>, grid, dim3(THREADS), lds, stream, e->d_engine, r, channels, \
+                               masks, mask_len, first_epoch + done, gain, o, cp, e->d_err);                     \
+            break;
+        PHAROS_WINDOW(PHAROS_SYNTH_IQ2)
+        PHAROS_WINDOW(PHAROS_SYNTH_IQ4)
+        PHAROS_WINDOW(PHAROS_SYNTH_IQ8)
+        PHAROS_WINDOW(PHAROS_SYNTH_IQ16)
+        PHAROS_WINDOW(PHAROS_SYNTH_F32)
+#undef PHAROS_WINDOW
+        }
         HIP_TRY(hipGetLastError(), ST_INTERNAL);
         done += n;
     }
@@ -466,7 +495,7 @@ struct pharos_synth_batch_handle {
     DevAdjust *d_adjust = nullptr;
     unsigned long long *d_seg_clip = nullptr;
     uint32_t nsegs = 0, adjust_cap = 0;
-    std::vector<uint32_t> job_first_seg, job_seg_count;
+    std::vector<uint32_t> job_first_seg, job_seg_count, seg_job;
     uint32_t *d_err = nullptr;
     hipEvent_t done = nullptr;
     std::vector<uint8_t> upload;  // host image of the batch tables, alive until the copy has run
@@ -504,7 +533,6 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     *out = nullptr;
     if (njobs == 0) { set_error("a batch needs at least one job"); return ST_ARG; }
     const int device = jobs[0].engine ? jobs[0].engine->device : 0;
-    uint32_t lds = 0;
     for (uint32_t j = 0; j < njobs; j++) {
         const pharos_synth_job &J = jobs[j];
         if (!J.engine || !J.out || (J.channels && !J.rows) || J.epochs == 0) { set_error("job %u is incomplete", j); return ST_ARG; }
@@ -512,22 +540,20 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
         if (J.gain_mode > 1) { set_error("job %u: gain mode %u is not defined", j, J.gain_mode); return ST_ARG; }
         if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT && !(J.constant_gain > 0.0f)) { set_error("job %u: constant gain must be positive", j); return ST_ARG; }
         if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE && !J.agc) { set_error("job %u: adaptive gain needs its AGC state", j); return ST_ARG; }
-        lds = std::max(lds, J.engine->view.lds_bytes);
     }
     HIP_TRY(hipSetDevice(device), ST_INTERNAL);
-    const uint32_t queue_words_offset = (lds + 15u) & ~15u;
-    lds = queue_words_offset + 16;
     pharos_synth_batch_handle *h = new (std::nothrow) pharos_synth_batch_handle();
     if (!h) { set_error("out of memory"); return ST_NOMEM; }
     h->device = device;
     h->stream = (hipStream_t)stream_;
     h->jobs.assign(jobs, jobs + njobs);
 
-    // Windows end where the AGC evaluates (multiples of 100 epochs, gain.zig:5,90-93).  Items run round by round:
-    // every job's first window, then every job's second, so a window's gain is ready long before it is needed.
+    // Windows end where the AGC evaluates (multiples of 100 epochs, gain.zig:5,90-93).  Jobs run in groups of one
+    // output format, one persistent launch per group (a launch carries only its format's quantiser).  Within a
+    // group items run round by round: every job's first window, then every job's second, so a window's gain is
+    // ready long before it is needed.
     struct Win { uint32_t job, off, epochs; };
     std::vector<std::vector<Win>> per_job(njobs);
-    size_t rounds = 0;
     for (uint32_t j = 0; j < njobs; j++) {
         const pharos_synth_job &J = jobs[j];
         uint32_t off = 0;
@@ -538,7 +564,14 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
             per_job[j].push_back({j, off, n});
             off += n;
         }
-        rounds = std::max(rounds, per_job[j].size());
+    }
+    struct Group { uint32_t format, lds; uint64_t begin, end; };
+    std::vector<Group> groups;
+    for (uint32_t j = 0; j < njobs; j++) {
+        const uint32_t f = jobs[j].engine->view.format;
+        bool seen = false;
+        for (const Group &g : groups) seen |= g.format == f;
+        if (!seen) groups.push_back({f, 0, 0, 0});
     }
     std::vector<DevSegment> segs;
     std::vector<uint32_t> seg_next;
@@ -546,9 +579,17 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     h->job_first_seg.assign(njobs, 0xffffffffu);
     h->job_seg_count.assign(njobs, 0);
     uint64_t items = 0;
-    for (size_t r = 0; r < rounds; r++) {
+    for (Group &g : groups) {
+      g.begin = items;
+      size_t rounds = 0;
+      for (uint32_t j = 0; j < njobs; j++) {
+          if (jobs[j].engine->view.format != g.format) continue;
+          rounds = std::max(rounds, per_job[j].size());
+          g.lds = std::max(g.lds, jobs[j].engine->view.lds_bytes);
+      }
+      for (size_t r = 0; r < rounds; r++) {
         for (uint32_t j = 0; j < njobs; j++) {
-            if (r >= per_job[j].size()) continue;
+            if (jobs[j].engine->view.format != g.format || r >= per_job[j].size()) continue;
             const Win &w = per_job[j][r];
             DevSegment s;
             s.first_item = items;
@@ -559,13 +600,17 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
             items += (uint64_t)s.epochs * s.tiles;
             const uint32_t idx = (uint32_t)segs.size();
             segs.push_back(s);
+            h->seg_job.push_back(j);
             seg_next.push_back(0xffffffffu);
             if (last_seg[j] != 0xffffffffu) seg_next[last_seg[j]] = idx;
             else h->job_first_seg[j] = idx;
             last_seg[j] = idx;
             h->job_seg_count[j]++;
         }
+      }
+      g.end = items;
     }
+    const uint32_t ngroups = (uint32_t)groups.size();
     const uint32_t nsegs = (uint32_t)segs.size();
     h->nsegs = nsegs;
     uint32_t adjust_cap = 1;
@@ -605,7 +650,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     size_t total = 0;
     auto carve = [&](size_t n) { size_t o = (total + 255) & ~(size_t)255; total = o + n; return Piece{o, n}; };
     Piece p_jobs = carve(sizeof(DevJob) * njobs), p_segs = carve(sizeof(DevSegment) * nsegs),
-          p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long)),
+          p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long) * ngroups),
           p_done = carve(sizeof(uint32_t) * nsegs), p_clip = carve(sizeof(unsigned long long) * nsegs),
           p_gain = carve(sizeof(float) * nsegs), p_ready = carve(sizeof(uint32_t) * nsegs),
           p_agc = carve(sizeof(DevAgc) * njobs), p_adj = carve(sizeof(DevAdjust) * (size_t)adjust_cap * njobs),
@@ -617,6 +662,10 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     memcpy(host.data() + p_gain.off, seg_gain.data(), p_gain.size);
     memcpy(host.data() + p_ready.off, seg_ready.data(), p_ready.size);
     memcpy(host.data() + p_agc.off, agc.data(), p_agc.size);
+    for (uint32_t g = 0; g < ngroups; g++) {
+        const unsigned long long q0 = groups[g].begin;
+        memcpy(host.data() + p_queue.off + g * sizeof q0, &q0, sizeof q0);
+    }
     void *base;
     int st = batch_alloc(h, &base, total);
     if (st) { batch_free(h); return st; }
@@ -633,7 +682,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.seg_next = (const uint32_t *)(b8 + p_next.off);
     S.nsegs = nsegs;
     S.total_items = items;
-    S.queue = (unsigned long long *)(b8 + p_queue.off);
+    S.queue = nullptr;
     S.tiles_done = (uint32_t *)(b8 + p_done.off);
     S.seg_clip = (unsigned long long *)(b8 + p_clip.off);
     S.seg_gain = (float *)(b8 + p_gain.off);
@@ -642,24 +691,44 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.adjust = (DevAdjust *)(b8 + p_adj.off);
     S.adjust_cap = adjust_cap;
     S.err = (uint32_t *)(b8 + p_err.off);
-    S.queue_words_offset = queue_words_offset;
+    S.queue_words_offset = 0;
     h->d_agc = S.agc;
     h->d_adjust = S.adjust;
     h->d_seg_clip = S.seg_clip;
     h->d_err = S.err;
 
-    uint32_t per_cu = tuning && tuning->workgroups_per_cu ? tuning->workgroups_per_cu : 0;
-    if (!per_cu) {
-        int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, synth_batch_kernel, THREADS, lds) != hipSuccess || occ <= 0) occ = 2;
-        per_cu = (uint32_t)occ;
+    for (uint32_t g = 0; g < ngroups; g++) {
+        const Group &G = groups[g];
+        BatchState Sg = S;
+        Sg.queue = (unsigned long long *)(b8 + p_queue.off) + g;
+        Sg.total_items = G.end;
+        Sg.queue_words_offset = (G.lds + 15u) & ~15u;
+        const uint32_t lds = Sg.queue_words_offset + 16;
+        const void *fn = nullptr;
+        switch (G.format) {
+        case PHAROS_SYNTH_IQ2: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ2>; break;
+        case PHAROS_SYNTH_IQ4: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ4>; break;
+        case PHAROS_SYNTH_IQ8: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ8>; break;
+        case PHAROS_SYNTH_IQ16: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ16>; break;
+        default: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_F32>; break;
+        }
+        uint32_t per_cu = tuning && tuning->workgroups_per_cu ? tuning->workgroups_per_cu : 0;
+        if (!per_cu) {
+            int occ = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, THREADS, lds) != hipSuccess || occ <= 0) occ = 2;
+            per_cu = (uint32_t)occ;
+        }
+        uint64_t grid = (uint64_t)cu_count(device) * per_cu;
+        if (grid > G.end - G.begin) grid = G.end - G.begin;
+        if (grid == 0) grid = 1;
+        void *args[] = {&Sg};
+        if (hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(THREADS), args, lds, h->stream) != hipSuccess) {
+            set_error("batch launch failed");
+            (void)hipStreamSynchronize(h->stream);
+            batch_free(h);
+            return ST_INTERNAL;
+        }
     }
-    uint64_t grid = (uint64_t)cu_count(device) * per_cu;
-    (void)0;
-    if (grid > items) grid = items;
-    if (grid == 0) grid = 1;
-    hipLaunchKernelGGL(synth_batch_kernel, dim3((uint32_t)grid), dim3(THREADS), lds, h->stream, S);
-    if (hipGetLastError() != hipSuccess) { set_error("batch launch failed"); batch_free(h); return ST_INTERNAL; }
     if (hipEventCreateWithFlags(&h->done, hipEventDisableTiming) != hipSuccess || hipEventRecord(h->done, h->stream) != hipSuccess) {
         set_error("event record failed");
         (void)hipStreamSynchronize(h->stream);
@@ -697,20 +766,8 @@ extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
         batch_free(h);
         return ST_INTERNAL;
     }
-    // Segments of job j are job_first_seg[j] then each seg_next; sum their clip counts by walking the rounds.
     std::vector<uint64_t> job_clip(njobs, 0);
-    {
-        // Rebuild the job of every segment in submission order (round-major, as built at submit).
-        std::vector<uint32_t> remaining = h->job_seg_count;
-        uint32_t s = 0;
-        while (s < h->nsegs) {
-            for (uint32_t j = 0; j < njobs && s < h->nsegs; j++) {
-                if (!remaining[j]) continue;
-                remaining[j]--;
-                job_clip[j] += seg_clip[s++];
-            }
-        }
-    }
+    for (uint32_t s = 0; s < h->nsegs; s++) job_clip[h->seg_job[s]] += seg_clip[s];
     for (uint32_t j = 0; j < njobs; j++) {
         const pharos_synth_job &J = h->jobs[j];
         if (J.clipped) *J.clipped = job_clip[j];

@@ -11,6 +11,19 @@
 
 namespace pharos {
 
+// Global-memory views of pointers the kernels read from memory: typed as global, their loads are global (not
+// flat) loads, which do not hold up the LDS accesses that follow them.
+template <class T> using gptr = const __attribute__((address_space(1))) T *;
+template <class T> __device__ __forceinline__ gptr<T> gmem(const T *p) { return (gptr<T>)p; }
+template <class T> using gmut = __attribute__((address_space(1))) T *;
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint4 gload_u4(const void *p) {
+    const u32x4_t v = *gmem(reinterpret_cast<const u32x4_t *>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // ------------------------------------------------------------------ f64 turn-table sine/cosine (rotor.zig:76-86)
 __device__ __forceinline__ void sincos64(double phi, const double *__restrict__ turn, double &c, double &s) {
     double t = phi - floor(phi);
@@ -21,7 +34,8 @@ __device__ __forceinline__ void sincos64(double phi, const double *__restrict__ 
     double r2 = r * r;
     double cr = 1.0 - r2 * (0.5 - r2 * (1.0 / 24.0 - r2 / 720.0));
     double sr = r * (1.0 - r2 * (1.0 / 6.0 - r2 * (1.0 / 120.0 - r2 / 5040.0)));
-    double ec = turn[2 * i], es = turn[2 * i + 1];
+    const f64x2_t e = *gmem(reinterpret_cast<const f64x2_t *>(turn + 2 * i));
+    const double ec = e.x, es = e.y;
     c = ec * cr - es * sr;
     s = es * cr + ec * sr;
 }
@@ -162,7 +176,7 @@ __device__ __forceinline__ float2 lds_load_f2(uint32_t a) {
 struct Lds {
     pharos_synth_row *rows;  // the chunk's rows (MAX_CHUNK)
     ChanRec *rec;
-    uint8_t *vt;      // chunk x VT_BYTES
+    uint8_t *vt;      // chunk x VT_BYTES: 16 float2 values per channel
     uint8_t *stage;   // chunk x w_slot
     float *sum_re, *sum_im;
     uint32_t *red;    // reduction words
@@ -191,20 +205,38 @@ __device__ __forceinline__ Lds lds_layout(uint8_t *smem, const DevEngine &E) {
     return L;
 }
 
-__device__ __forceinline__ uint8_t mask_at(const ChanRec &R, const uint8_t *masks, uint64_t mask_len, uint32_t k) {
-    if (k >= R.mask_count) return 0;
-    uint64_t at = (uint64_t)R.mask_offset + k;
-    return at < mask_len ? masks[at] : 0;
+// x / d and x % d for a 32-bit divisor; the common case of a 32-bit dividend takes the short hardware sequence.
+__device__ __forceinline__ void divmod32(uint64_t x, uint32_t d, uint64_t &q, uint32_t &r) {
+    if ((x >> 32) == 0) {
+        const uint32_t q32 = (uint32_t)x / d;
+        q = q32;
+        r = (uint32_t)x - q32 * d;
+    } else {
+        q = x / d;
+        r = (uint32_t)(x - q * d);
+    }
 }
 
-// Setup stage A, one thread per channel of the chunk: the channel's code edge, staged window and shape for this
-// tile, from its row (already in LDS).  No rotor work here.
-__device__ void chan_core(ChanRec &R, const pharos_synth_row &row, const DevEngine &E, uint32_t n0, uint32_t tile_len,
-                          uint64_t mask_len, uint32_t *err) {
+// Setup of one channel for one tile, one thread per channel of the chunk, from its row (already in LDS): the
+// code edge, staged window, shape and symbol masks, and Rotor.init (rotor.zig:21-35) -- the lane powers by the
+// reference's f64 recursion and the f32 16-sample step.  Every global load it needs is independent of the
+// others' results but the table's period, so they are in flight together.
+__device__ __forceinline__ void chan_setup(ChanRec &R, float2 *vt, const pharos_synth_row &row, const DevEngine &E,
+                                           uint32_t n0, uint32_t tile_len, const uint8_t *masks, uint64_t mask_len,
+                                           uint32_t *err) {
     R.active = 0;
     if (row.table == PHAROS_SYNTH_INACTIVE) return;
     if (row.table >= E.ntables) { atomicOr(err, ERR_TABLE); return; }
-    const DevTable &T = E.tables[row.table];
+    const auto &T = gmem(E.tables)[row.table];
+    const uint32_t per = T.per, w_max = T.w_max, ext_len = T.ext_len;
+    const uint32_t axis = T.axis, tdm = T.tdm_periods, count = T.count;
+    const uint8_t *ext = T.ext;
+    double vre[8], vim[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) { vre[k] = T.vre[k]; vim[k] = T.vim[k]; }
+    double w1c, w1s, wc, ws;
+    sincos64(row.carrier_step, E.turn, w1c, w1s);
+    sincos64(16.0 * row.carrier_step, E.turn, wc, ws);
     if (!(row.entries_per_sample > 0.0) || !(row.entries_per_sample < 64.0) || !(row.entry_phase0 >= 0.0)) {
         atomicOr(err, ERR_TABLE);
         return;
@@ -218,8 +250,7 @@ __device__ void chan_core(ChanRec &R, const pharos_synth_row &row, const DevEngi
     const uint64_t P0 = phase_q + (uint64_t)n0 * eps_q;
     const uint64_t Pl = phase_q + (uint64_t)(n0 + tile_len - 1) * eps_q;
     const uint64_t e_lo = first + (P0 >> 32);
-    const uint32_t per = T.per;
-    const uint32_t shape = T.tdm_periods ? SHAPE_TDM : (row.lead_bits ? SHAPE_LEAD : SHAPE_PLAIN);
+    const uint32_t shape = tdm ? SHAPE_TDM : (row.lead_bits ? SHAPE_LEAD : SHAPE_PLAIN);
     uint32_t lead_whole = 0, lead_q = 0;
     if (shape == SHAPE_LEAD) {
         if (!(row.lead_entries >= 0.0) || !(row.lead_entries < (double)LEAD_ALLOW)) { atomicOr(err, ERR_STAGE); return; }
@@ -227,43 +258,52 @@ __device__ void chan_core(ChanRec &R, const pharos_synth_row &row, const DevEngi
         lead_whole = (uint32_t)lw;
         lead_q = frac_q(row.lead_entries - lw);  // chip.zig:162-165
     }
-    const uint64_t W = (Pl >> 32) - (P0 >> 32) + 2 + lead_whole;
-    const uint64_t k_lo = e_lo / per, e0 = e_lo % per;
-    if (W > T.w_max || e0 + W > T.ext_len) { atomicOr(err, ERR_STAGE); return; }
+    const uint64_t span = (Pl >> 32) - (P0 >> 32);
+    const uint64_t W = span + 2 + lead_whole;
+    uint64_t k_lo;
+    uint32_t e0;
+    divmod32(e_lo, per, k_lo, e0);
+    if (W > w_max || e0 + W > ext_len) { atomicOr(err, ERR_STAGE); return; }
     // The last entry a live sample reads (the lead adds at most one carry), and the period it falls in: the
     // reference's mask slice covers masksNeeded periods (parameters.zig:25-29), so a row reading past the host's
     // mask bytes is a host error.
-    const uint64_t last_rel = (Pl >> 32) - (P0 >> 32) + lead_whole + (shape == SHAPE_LEAD ? 1u : 0u);
-    const uint64_t w_used = (e0 + last_rel) / per;
+    const uint64_t last_rel = span + lead_whole + (shape == SHAPE_LEAD ? 1u : 0u);
+    uint64_t w_used;
+    uint32_t unused;
+    divmod32(e0 + last_rel, per, w_used, unused);
+    (void)unused;
     if (k_lo + w_used >= row.mask_count || (uint64_t)row.mask_offset + k_lo + w_used >= mask_len) {
         atomicOr(err, ERR_MASKS);
         return;
     }
+    // The symbol masks of the (at most four) periods the window touches; zero past the last one it reads.
+    uint32_t m4 = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; w++)
+        if (w <= w_used) m4 |= (uint32_t)gmem(masks)[(uint64_t)row.mask_offset + k_lo + w] << (8 * w);
+    // The channel's values scaled by its amplitude (kernel.zig:104-110); entries from count on stay zero.
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++)
+        vt[k] = k < count ? make_float2((float)((double)row.amp * vre[k]), (float)((double)row.amp * vim[k]))
+                          : make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t k = 8; k < 16; k++) vt[k] = make_float2(0.0f, 0.0f);
+    const uint32_t head = e0 & 3u;
+    R.src = reinterpret_cast<const uint32_t *>(ext + (e0 - head));
+    R.ndw = (uint32_t)((head + W + 3) / 4);
+    R.m4 = m4;
+    R.wide_mask = (m4 & 0xf8f8f8f8u) != 0;
     R.shape = shape;
-    R.axis = T.axis;
+    R.axis = axis;
     R.table = row.table;
     R.P0 = P0;
     R.eps_q = eps_q;
-    R.e0 = (uint32_t)e0;
+    R.e0 = e0;
     R.k_lo = (uint32_t)k_lo;
     R.W = (uint32_t)W;
     R.lead_q = lead_q;
     R.lead_whole = lead_whole;
     R.lead_bits = row.lead_bits;
-    R.mask_offset = row.mask_offset;
-    R.mask_count = row.mask_count;
-    R.phase0 = row.carrier_phase0;
-    R.step = row.carrier_step;
-    R.active = 1;
-}
-
-// Setup stage B, one thread per channel: Rotor.init (rotor.zig:21-35), the lane powers by the same f64 recursion
-// and the f32 16-sample step.
-__device__ void chan_rotor(ChanRec &R, const double *turn) {
-    if (!R.active) return;
-    double w1c, w1s, wc, ws;
-    sincos64(R.step, turn, w1c, w1s);
-    sincos64(16.0 * R.step, turn, wc, ws);
     double ac = 1.0, as = 0.0;
     R.pw_c[0] = 1.0;
     R.pw_s[0] = 0.0;
@@ -278,6 +318,7 @@ __device__ void chan_rotor(ChanRec &R, const double *turn) {
     }
     R.w_re = (float)wc;
     R.w_im = (float)ws;
+    R.active = 1;
 }
 
 // The per-sample loop of one channel for one thread: 32 samples at stride 16 of one rotor block.
@@ -286,39 +327,64 @@ template <uint32_t AXIS, bool LEAD>
 __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos, uint64_t step,
                                                 uint32_t vt_addr, float cre, float cim, float wre, float wim,
                                                 uint64_t lead_add, uint32_t lead_bits) {
+    // Eight samples at a time: their entry reads, then their value reads, are in flight together before the
+    // arithmetic that needs them, so LDS latency overlaps instead of adding up sample by sample.
+    constexpr uint32_t B = 8;
     const float nwim = -wim;
 #pragma unroll
-    for (uint32_t k = 0; k < STEPS; k++) {
-        const uint32_t a = (uint32_t)(pos >> 32);
-        uint32_t va;
+    for (uint32_t k0 = 0; k0 < STEPS; k0 += B) {
+        uint32_t va[B];
         if (LEAD) {
             // chip.zig:143-159 / kernel.zig:233-234: the leading bits come from the entry the lead reaches.
-            const uint32_t al = (uint32_t)((pos + lead_add) >> 32);
-            const uint32_t bm = lds_load_u8(a), bl = lds_load_u8(al);
-            const uint32_t idx = (bl & lead_bits) | (bm & ~lead_bits);
-            va = vt_addr + idx * 8u;
+            uint32_t bm[B], bl[B];
+#pragma unroll
+            for (uint32_t k = 0; k < B; k++) {
+                const uint64_t p = pos + (uint64_t)k * step;
+                bm[k] = lds_load_u8((uint32_t)(p >> 32));
+                bl[k] = lds_load_u8((uint32_t)((p + lead_add) >> 32));
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < B; k++) va[k] = vt_addr + ((bl[k] & lead_bits) | (bm[k] & ~lead_bits)) * 8u;
         } else {
-            va = vt_addr + lds_load_u8(a);
+            uint32_t e[B];
+#pragma unroll
+            for (uint32_t k = 0; k < B; k++) e[k] = lds_load_u8((uint32_t)((pos + (uint64_t)k * step) >> 32));
+#pragma unroll
+            for (uint32_t k = 0; k < B; k++) va[k] = vt_addr + e[k];
         }
-        // kernel.zig:241-258
-        if (AXIS == AXIS_BOTH) {
-            const float2 v = lds_load_f2(va);
-            ar[k] = fmaf(v.x, cre, fmaf(-v.y, cim, ar[k]));
-            ai[k] = fmaf(v.x, cim, fmaf(v.y, cre, ai[k]));
-        } else if (AXIS == AXIS_RE) {
-            const float v = lds_load_f32(va);
-            ar[k] = fmaf(v, cre, ar[k]);
-            ai[k] = fmaf(v, cim, ai[k]);
-        } else {
-            const float v = lds_load_f32(va + 4);
-            ar[k] = fmaf(-v, cim, ar[k]);
-            ai[k] = fmaf(v, cre, ai[k]);
+        pos += (uint64_t)B * step;
+        float vx[B], vy[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) {
+            if (AXIS == AXIS_BOTH) {
+                const float2 v = lds_load_f2(va[k]);
+                vx[k] = v.x;
+                vy[k] = v.y;
+            } else if (AXIS == AXIS_RE) {
+                vx[k] = lds_load_f32(va[k]);
+            } else {
+                vy[k] = lds_load_f32(va[k] + 4);
+            }
         }
-        // Rotor.advance (rotor.zig:52-58)
-        const float t = fmaf(cre, wre, cim * nwim);
-        cim = fmaf(cre, wim, cim * wre);
-        cre = t;
-        pos += step;
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++) {
+            const uint32_t i = k0 + k;
+            // kernel.zig:241-258
+            if (AXIS == AXIS_BOTH) {
+                ar[i] = fmaf(vx[k], cre, fmaf(-vy[k], cim, ar[i]));
+                ai[i] = fmaf(vx[k], cim, fmaf(vy[k], cre, ai[i]));
+            } else if (AXIS == AXIS_RE) {
+                ar[i] = fmaf(vx[k], cre, ar[i]);
+                ai[i] = fmaf(vx[k], cim, ai[i]);
+            } else {
+                ar[i] = fmaf(-vy[k], cim, ar[i]);
+                ai[i] = fmaf(vy[k], cre, ai[i]);
+            }
+            // Rotor.advance (rotor.zig:52-58)
+            const float t = fmaf(cre, wre, cim * nwim);
+            cim = fmaf(cre, wim, cim * wre);
+            cre = t;
+        }
     }
 }
 
@@ -392,14 +458,14 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
                 }
             }
         }
-        uint8_t *d = dst + bytes_for(F, g0);
+        const gmut<uint8_t> d = (gmut<uint8_t>)(dst + bytes_for(F, g0));
         if (g0 + G <= valid && aligned) {
-            if (UB == 4) *reinterpret_cast<uint32_t *>(d) = w[0];
-            else if (UB == 8) *reinterpret_cast<uint2 *>(d) = make_uint2(w[0], w[1 % NW]);
-            else if (UB == 16) *reinterpret_cast<uint4 *>(d) = make_uint4(w[0], w[1 % NW], w[2 % NW], w[3 % NW]);
+            if (UB == 4) *reinterpret_cast<gmut<uint32_t>>(d) = w[0];
+            else if (UB == 8) *reinterpret_cast<gmut<u32x2_t>>(d) = u32x2_t{w[0], w[1 % NW]};
+            else if (UB == 16) *reinterpret_cast<gmut<u32x4_t>>(d) = u32x4_t{w[0], w[1 % NW], w[2 % NW], w[3 % NW]};
             else {
-                reinterpret_cast<uint4 *>(d)[0] = make_uint4(w[0], w[1 % NW], w[2 % NW], w[3 % NW]);
-                reinterpret_cast<uint4 *>(d)[1] = make_uint4(w[4 % NW], w[5 % NW], w[6 % NW], w[7 % NW]);
+                reinterpret_cast<gmut<u32x4_t>>(d)[0] = u32x4_t{w[0], w[1 % NW], w[2 % NW], w[3 % NW]};
+                reinterpret_cast<gmut<u32x4_t>>(d)[1] = u32x4_t{w[4 % NW], w[5 % NW], w[6 % NW], w[7 % NW]};
             }
         } else {
             const uint32_t nb = g0 + G <= valid ? UB : bytes_for(F, valid - g0);
@@ -409,6 +475,61 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
         }
     }
     return clip;
+}
+
+// Staged dword d of a channel, byte by byte: the entry's value index, as 8 x the index (the value's byte offset in
+// the channel's table) or, for a lead channel, the bare index the lead combines (kernel.zig:227-235).  Plain: the
+// state with its period's symbol mask applied (kernel.zig:183).  TDM (kernel.zig:197-203): odd entries carry
+// 2 | the pilot chip of their period, even ones the state with the mask's low bit.  An index past the last value
+// stages as 8, a zero entry, where the reference would index out of its table.
+__device__ __forceinline__ uint32_t stage_bytes(const ChanRec &R, const pharos_synth_row &row, const DevEngine &E,
+                                                uint32_t v, uint32_t d) {
+    const uint32_t head = R.e0 & 3u;
+    uint32_t o = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+        const uint32_t x = (v >> (8 * q)) & 0xffu;
+        const uint32_t w = x >> 6, st = (x >> 3) & 7u;
+        const uint32_t m = (R.m4 >> (8 * w)) & 0xffu;
+        uint32_t idx = st ^ m;
+        if (R.shape == SHAPE_TDM) {
+            const auto &T = gmem(E.tables)[R.table];
+            const int32_t j = (int32_t)(4 * d + q) - (int32_t)head;
+            const uint32_t e = R.e0 + (uint32_t)(j < 0 ? 0 : j) - w * T.per;
+            if (j >= 0 && (e & 1u)) {
+                const uint64_t period = row.period0 + R.k_lo + w;
+                const uint32_t pi = (uint32_t)(period % T.tdm_periods) * T.tdm_half + e / 2;
+                const uint32_t chip = pi < T.pilot_len ? gmem(T.pilot)[pi] : 0u;
+                idx = 2u | chip;
+            } else {
+                idx = st ^ (m & 1u);
+            }
+        }
+        idx = idx < 8u ? idx : 8u;
+        o |= (R.shape == SHAPE_LEAD ? idx : 8u * idx) << (8 * q);
+    }
+    return o;
+}
+
+// The engine descriptor by value, read through a global view (the pointer may come from memory).
+__device__ __forceinline__ DevEngine load_engine(const DevEngine *p) {
+    const auto &g = *gmem(p);
+    DevEngine e;
+    e.tables = g.tables;
+    e.turn = g.turn;
+    e.seed = g.seed;
+    e.bpe = g.bpe;
+    e.ntables = g.ntables;
+    e.spe = g.spe;
+    e.format = g.format;
+    e.noise = g.noise;
+    e.tiles = g.tiles;
+    e.w_slot = g.w_slot;
+    e.chunk = g.chunk;
+    e.lds_bytes = g.lds_bytes;
+    e.out_bytes = g.out_bytes;
+    e.pad = 0;
+    return e;
 }
 
 struct TileArgs {
@@ -423,12 +544,12 @@ struct TileArgs {
 };
 
 
-// One work item.  AfterSums is called by every thread when the last chunk's sums are done (the persistent kernel
-// fetches its next item there); GainFn by every thread before quantising (the persistent kernel waits there for the
-// AGC of the previous window).  Returns the clipped components of the tile.
-template <class GainFn, class AfterSums>
+// One work item of an engine whose output format is F.  AfterSums is called by every thread when the last chunk's
+// sums are done (the persistent kernel fetches its next item there); GainFn by every thread before quantising (the
+// persistent kernel waits there for the AGC of the previous window).  Returns the clipped components of the tile.
+template <uint32_t F, class GainFn, class AfterSums>
 __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, AfterSums after_sums, uint32_t *err) {
-    const DevEngine &E = *A.E;
+    const DevEngine E = load_engine(A.E);
     const uint32_t tid = threadIdx.x;
     const uint32_t b = tid / LANES16, l = tid % LANES16;
     const uint32_t n0 = A.tile * TILE;
@@ -448,80 +569,60 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
     {
         const uint32_t nc = min(E.chunk, A.channels);
         if (tid < nc * row_words)
-            reinterpret_cast<uint4 *>(L.rows)[tid] = reinterpret_cast<const uint4 *>(A.rows)[tid];
+            reinterpret_cast<uint4 *>(L.rows)[tid] = gload_u4(reinterpret_cast<const uint4 *>(A.rows) + tid);
     }
     __syncthreads();
     for (uint32_t c0 = 0; c0 < A.channels; c0 += E.chunk) {
         const uint32_t nc = min(E.chunk, A.channels - c0);
-        // Stage A: per-channel edge, window and shape (LDS and table descriptors only).
-        if (tid < nc) chan_core(L.rec[tid], L.rows[tid], E, n0, tile_len, A.mask_len, err);
-        __syncthreads();
-        // Stage B: one round of independent global loads.  Lane powers (serial, one thread per channel) ...
-        if (tid < nc) chan_rotor(L.rec[tid], E.turn);
-        // ... block seeds, sincos64 of each block's start (Rotor.seed, rotor.zig:37-38) ...
+        // Stage A.  Block seeds, sincos64 of each block's start (Rotor.seed, rotor.zig:37-38), on every thread;
+        // the per-channel setup on one thread per channel.
         for (uint32_t i = tid; i < nc * TILE_BLOCKS; i += THREADS) {
-            ChanRec &R = L.rec[i / TILE_BLOCKS];
+            const pharos_synth_row &row = L.rows[i / TILE_BLOCKS];
+            if (row.table == PHAROS_SYNTH_INACTIVE) continue;
             const uint32_t bb = i % TILE_BLOCKS;
-            if (!R.active) continue;
             double c, s;
-            sincos64(R.phase0 + (double)(n0 + bb * RESEED) * R.step, E.turn, c, s);
-            R.c0_c[bb] = c;
-            R.c0_s[bb] = s;
+            sincos64(row.carrier_phase0 + (double)(n0 + bb * RESEED) * row.carrier_step, E.turn, c, s);
+            L.rec[i / TILE_BLOCKS].c0_c[bb] = c;
+            L.rec[i / TILE_BLOCKS].c0_s[bb] = s;
         }
-        // ... value tables, the symbol masks folded in per period offset (kernel.zig:104-110,183,202) ...
-        for (uint32_t i = tid; i < nc * 32; i += THREADS) {
-            const uint32_t ch = i / 32, w = (i / 8) % 4, s = i % 8;
-            const ChanRec &R = L.rec[ch];
-            if (!R.active) continue;
-            const DevTable &T = E.tables[R.table];
-            const float amp = L.rows[ch].amp;
-            uint32_t idx;
-            if (R.shape == SHAPE_LEAD) {
-                if (w != 0) continue;
-                idx = s;
-            } else {
-                const uint32_t m = mask_at(R, A.masks, A.mask_len, R.k_lo + w);
-                idx = R.shape == SHAPE_TDM ? (s < 2 ? s ^ (m & 1u) : s) : (s ^ m);
-            }
-            float2 v = make_float2(0.0f, 0.0f);
-            if (idx < T.count) v = make_float2((float)((double)amp * T.vre[idx]), (float)((double)amp * T.vim[idx]));
-            *reinterpret_cast<float2 *>(L.vt + ch * VT_BYTES + (w * 8 + s) * 8) = v;
-        }
-        // ... and each channel's entries of this tile, staged from its extended table.
-        for (uint32_t ch = 0; ch < nc; ch++) {
-            const ChanRec &R = L.rec[ch];
-            if (!R.active) continue;
-            const DevTable &T = E.tables[R.table];
-            const uint32_t head = R.e0 & 3u;
-            const uint32_t ndw = (head + R.W + 3) / 4;
-            const uint32_t *src = reinterpret_cast<const uint32_t *>(T.ext + (R.e0 - head));
-            uint32_t *dst = reinterpret_cast<uint32_t *>(L.stage + ch * E.w_slot);
-            for (uint32_t d = tid; d < ndw; d += THREADS) {
-                uint32_t v = src[d];
-                if (R.shape != SHAPE_PLAIN) {
-                    uint32_t o = 0;
+        if (tid < nc)
+            chan_setup(L.rec[tid], reinterpret_cast<float2 *>(L.vt + tid * VT_BYTES), L.rows[tid], E, n0, tile_len,
+                       A.masks, A.mask_len, err);
+        __syncthreads();
+        // Stage B.  Every channel's entries of this tile, staged from its extended table as value offsets, by a
+        // group of STAGE_GROUP threads per channel that reads the channel's descriptor once and keeps eight loads
+        // in flight.  With every mask below 8 a dword's four bytes map at once: v_perm picks each byte's period
+        // mask by its period offset.
+        {
+            constexpr uint32_t STAGE_GROUP = THREADS / MAX_CHUNK;
+            const uint32_t c = tid / STAGE_GROUP, j = tid % STAGE_GROUP;
+            const ChanRec &R = L.rec[c < nc ? c : 0];
+            const uint32_t ndw = c < nc && R.active ? R.ndw : 0u;
+            if (ndw) {
+                const gptr<uint32_t> src = gmem(R.src);
+                uint32_t *dst = reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot);
+                if (R.wide_mask || R.shape == SHAPE_TDM) {
+                    for (uint32_t d = j; d < ndw; d += STAGE_GROUP) dst[d] = stage_bytes(R, L.rows[c], E, src[d], d);
+                } else {
+                    const bool lead = R.shape == SHAPE_LEAD;
+                    const uint32_t m = lead ? R.m4 : R.m4 << 3;
+                    const uint32_t keep = lead ? 0x07070707u : 0x38383838u;
+                    const uint32_t sh = lead ? 3u : 0u;
+                    for (uint32_t d0 = j; d0 < ndw; d0 += 8 * STAGE_GROUP) {
+                        uint32_t v[8];
 #pragma unroll
-                    for (uint32_t q = 0; q < 4; q++) {
-                        uint32_t x = (v >> (8 * q)) & 0xffu;
-                        const uint32_t w = x >> 6;
-                        if (R.shape == SHAPE_LEAD) {
-                            x = ((x >> 3) & 7u) ^ mask_at(R, A.masks, A.mask_len, R.k_lo + w);
-                        } else {
-                            // TDM (kernel.zig:197-203): odd entries carry 2 | the pilot chip of their period.
-                            const int32_t j = (int32_t)(4 * d + q) - (int32_t)head;
-                            const uint32_t e = R.e0 + (uint32_t)(j < 0 ? 0 : j) - w * T.per;
-                            if (j >= 0 && (e & 1u)) {
-                                const uint64_t period = L.rows[ch].period0 + R.k_lo + w;
-                                const uint32_t pi = (uint32_t)(period % T.tdm_periods) * T.tdm_half + e / 2;
-                                const uint32_t chip = pi < T.pilot_len ? T.pilot[pi] : 0u;
-                                x = 8u * (2u | chip) + 64u * w;
-                            }
+                        for (uint32_t k = 0; k < 8; k++) {
+                            const uint32_t d = d0 + k * STAGE_GROUP;
+                            v[k] = d < ndw ? src[d] : 0u;
                         }
-                        o |= x << (8 * q);
+#pragma unroll
+                        for (uint32_t k = 0; k < 8; k++) {
+                            const uint32_t d = d0 + k * STAGE_GROUP;
+                            if (d < ndw)
+                                dst[d] = ((v[k] >> sh) & keep) ^ __builtin_amdgcn_perm(m, m, (v[k] >> 6) & 0x03030303u);
+                        }
                     }
-                    v = o;
                 }
-                dst[d] = v;
             }
         }
         __syncthreads();
@@ -530,7 +631,7 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
         const uint32_t nc_next = c1 < A.channels ? min(E.chunk, A.channels - c1) : 0u;
         uint4 next_row = make_uint4(0, 0, 0, 0);
         const bool fetch = tid < nc_next * row_words;
-        if (fetch) next_row = reinterpret_cast<const uint4 *>(A.rows + c1)[tid];
+        if (fetch) next_row = gload_u4(reinterpret_cast<const uint4 *>(A.rows + c1) + tid);
         // The channel sums.
         if (block_live) {
             for (uint32_t ch = 0; ch < nc; ch++) {
@@ -580,14 +681,7 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
     const uint32_t valid = remain >= TILE ? TILE : remain;
     const uint64_t abs0 = A.epoch * (uint64_t)E.spe + n0;
     uint8_t *dst = A.out_epoch + bytes_for(E.format, n0);
-    uint32_t clip;
-    switch (E.format) {
-    case PHAROS_SYNTH_IQ2: clip = epilogue<PHAROS_SYNTH_IQ2>(L, E, valid, abs0, gain, dst); break;
-    case PHAROS_SYNTH_IQ4: clip = epilogue<PHAROS_SYNTH_IQ4>(L, E, valid, abs0, gain, dst); break;
-    case PHAROS_SYNTH_IQ8: clip = epilogue<PHAROS_SYNTH_IQ8>(L, E, valid, abs0, gain, dst); break;
-    case PHAROS_SYNTH_IQ16: clip = epilogue<PHAROS_SYNTH_IQ16>(L, E, valid, abs0, gain, dst); break;
-    default: clip = epilogue<PHAROS_SYNTH_F32>(L, E, valid, abs0, gain, dst); break;
-    }
+    uint32_t clip = epilogue<F>(L, E, valid, abs0, gain, dst);
 
     // Clip count of the tile.
 #pragma unroll
