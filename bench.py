@@ -60,12 +60,17 @@ def cpu_baseline(scenario, seconds=10.0, threads=None):
                       f"C restatement of kernel.zig on {threads} threads, constant gain, {dt:.1f} s wall"}
 
 
-def traffic_from_profiles():
-    path = os.path.join(ROOT, "profiles", "traffic_c1.json")
+def profiled(epochs):
+    """(HBM bytes of one launch over `epochs` epochs, VALU busy %) from the batched kernel's counter runs: the
+    per-epoch FETCH_SIZE + WRITE_SIZE, each scaled by the counter's reading on a copy of known size
+    (profiles/synth_batch_kernel_c1_traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "synth_batch_kernel_c1_traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        d = json.load(f)
+    per_epoch = d.get("hbm_bytes_per_epoch")
+    return (None if per_epoch is None else per_epoch * epochs), d.get("valu_busy_pct")
 
 
 def main():
@@ -79,6 +84,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", action="store_true", help="also run the Poisson serving measurement")
     ap.add_argument("--workgroups-per-cu", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device three times (a known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (runs shard by scenario or epoch "
@@ -90,6 +97,13 @@ def main():
     from pharos_amd import _lib
     import ctypes as C
     require_gpu()
+    if args.copy_calibration:
+        a, b = DeviceBuffer(args.copy_calibration), DeviceBuffer(args.copy_calibration)
+        for _ in range(3):
+            _lib.check(_lib.lib().pharos_synth_memcpy(b.ptr, a.ptr, args.copy_calibration, 2))
+        _lib.lib().pharos_synth_device_sync(0)
+        print(json.dumps({"copied_bytes": args.copy_calibration, "copies": 3}), flush=True)
+        return
 
     sc = S.c1(seed=1, epochs=args.epochs)
     nch = len(sc.channels)
@@ -153,6 +167,7 @@ def main():
     plain_value = cs_per_step / (sum(plain_t) / len(plain_t)) if plain_t else None
 
     achieved = flops_per_step / kernel_s / 1e12
+    traffic, valu_busy = profiled(sc.epochs)
     out_bytes = sc.epochs * sc.bytes_per_epoch
     line = {
         "metric": METRIC,
@@ -171,7 +186,7 @@ def main():
                    "format": "IQ4", "epochs": sc.epochs, "noise": "counter", "gain": "adaptive (device AGC)",
                    "form": "batched persistent launch", "channel_samples_per_step": cs_per_step},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profiles(),
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "valu_busy_pct_profiled": valu_busy,
                      "kernel": "synth_batch_kernel", "kernel_ms": kernel_s * 1e3,
                      "flops_per_launch": flops_per_step, "hbm_write_gbs": out_bytes / kernel_s / 1e9,
                      "hbm_write_frac": out_bytes / kernel_s / 1e9 / PEAK_HBM_GBS},
