@@ -297,3 +297,29 @@ def test_window_is_thread_count_invariant():
     a = O.synth_window(tabs, rows, masks, sc.samples_per_epoch, sc.format_id, 0, 9, 0, 1.0, threads=1)
     b = O.synth_window(tabs, rows, masks, sc.samples_per_epoch, sc.format_id, 0, 9, 0, 1.0, threads=8)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_code_position_error_against_fp64_over_an_epoch():
+    """Q32.32 code position (chip.zig:12-29) against the exact position entry_phase0 + n * eps over a whole
+    122 880-sample epoch: within (n + 1) * 2^-33 entries, so at most 1.43e-5 entries (chips for one entry per
+    chip); the entry index equals floor of the exact position away from such ties."""
+    rng = np.random.default_rng(7)
+    worst, ties = 0.0, 0
+    for _ in range(100):
+        ep0, eps = float(rng.uniform(0, 10230)), float(rng.uniform(0.001, 3.0))
+        fl = math.floor(ep0)
+        phase_q = min(int(round((ep0 - fl) * 2**32)), 2**32 - 1)
+        eps_q = int(round(eps * 2**32))
+        for k in list(rng.integers(0, 122_880, 200)) + [122_879]:
+            k = int(k)
+            exact = np.longdouble(ep0) + np.longdouble(k) * np.longdouble(eps)
+            q = fl + (phase_q + k * eps_q) / 2**32
+            err = abs(float(np.longdouble(q) - exact))
+            worst = max(worst, err)
+            assert err <= (k + 1) * 2.0**-33 + 1e-12
+            index = fl + ((phase_q + k * eps_q) >> 32)
+            if abs(float(exact - np.floor(exact))) > 2e-5:
+                assert index == int(np.floor(exact))
+            else:
+                ties += 1
+    assert worst < 1.5e-5
