@@ -5,10 +5,15 @@ GPS L1C/A+L5, Galileo E1+E5a, BeiDou B1I+B3I and GLONASS L1OF at 61.44 Msps, IQ4
 adaptive AGC -- rows and masks resident in HBM before the timed region, the 3.69 GB of IQ4 output written to HBM.
 
   value        channel-samples/s of the batched form (one persistent launch per step, device AGC)
+  graph        the same run as per-window launches with the AGC step on the device, captured once as a HIP graph
   plain        the same run as per-window plain launches with the host AGC in between
   roofline     FP32 VALU roofline of the synthesis kernel: algorithmic flops per launch / launch time
   cpu_baseline the C oracle (a restatement of the reference kernel) on the host cores, on a bounded sample
   serving      p50/p99 latency of a Poisson request stream (--serve)
+
+--workload c3 instead measures the uneven batch: 256 seeded scenarios (4-48 channels, 4.092-61.44 Msps,
+IQ2-IQ16, durations cut to --c3-epochs-cap epochs) as one batched launch per format, against per-scenario plain
+launches and per-scenario graphs.
 """
 import argparse
 import json
@@ -73,38 +78,63 @@ def profiled(epochs):
     return (None if per_epoch is None else per_epoch * epochs), d.get("valu_busy_pct")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--epochs", type=int, default=60_000, help="epochs per run (the configuration has 60 000)")
-    ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve", action="store_true", help="also run the Poisson serving measurement")
-    ap.add_argument("--workgroups-per-cu", type=int, default=0)
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
-                    help="only copy BYTES device-to-device three times (a known-size reference for HBM counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (runs shard by scenario or epoch "
-              f"range with no exchange; see the design notes)", file=sys.stderr)
-        sys.exit(2)
+def _reset(job):
+    job.agc.gain, job.agc.clipped, job.agc.components, job.agc.adjustments = 1.0, 0, 0, 0
+    job.clipped.value = 0
 
-    from pharos_amd import scenario as S
-    from pharos_amd.engine import Job, run_batch, DeviceBuffer, Engine, Event, Stream, require_gpu
-    from pharos_amd import _lib
+
+def timed_batch(jobs, workgroups_per_cu, stream):
+    """(wall seconds, kernel seconds by HIP events on the launch stream) of one batched launch over `jobs`."""
     import ctypes as C
-    require_gpu()
-    if args.copy_calibration:
-        a, b = DeviceBuffer(args.copy_calibration), DeviceBuffer(args.copy_calibration)
-        for _ in range(3):
-            _lib.check(_lib.lib().pharos_synth_memcpy(b.ptr, a.ptr, args.copy_calibration, 2))
-        _lib.lib().pharos_synth_device_sync(0)
-        print(json.dumps({"copied_bytes": args.copy_calibration, "copies": 3}), flush=True)
-        return
+    from pharos_amd import _lib
+    from pharos_amd.engine import Event
+    L = _lib.lib()
+    for j in jobs:
+        _reset(j)
+    arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
+    tuning = _lib.SynthTuning(workgroups_per_cu, (C.c_uint32 * 3)())
+    e0, e1 = Event(), Event()
+    h = C.c_void_p()
+    L.pharos_synth_device_sync(0)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    _lib.check(L.pharos_synth_batch_submit(arr, len(jobs), C.byref(tuning), stream.handle, C.byref(h)))
+    e1.record(stream)
+    _lib.check(L.pharos_synth_batch_wait(h))
+    L.pharos_synth_device_sync(0)
+    return time.perf_counter() - t0, e0.elapsed_ms(e1) / 1e3
 
+
+def timed_graphs(graphs, stream):
+    from pharos_amd import _lib
+    L = _lib.lib()
+    for g in graphs:
+        _reset(g.job)
+    L.pharos_synth_device_sync(0)
+    t0 = time.perf_counter()
+    for g in graphs:
+        g.launch(stream)
+    for g in graphs:
+        g.finish(stream)
+    L.pharos_synth_device_sync(0)
+    return time.perf_counter() - t0
+
+
+def timed_plain(jobs, stream):
+    from pharos_amd import _lib
+    from pharos_amd.engine import plain_job
+    L = _lib.lib()
+    L.pharos_synth_device_sync(0)
+    t0 = time.perf_counter()
+    for j in jobs:
+        plain_job(j, stream)
+    L.pharos_synth_device_sync(0)
+    return time.perf_counter() - t0
+
+
+def bench_c1(args):
+    from pharos_amd import scenario as S
+    from pharos_amd.engine import Job, Graph, Stream
     sc = S.c1(seed=1, epochs=args.epochs)
     nch = len(sc.channels)
     rows, masks = sc.window(0, sc.epochs)
@@ -113,57 +143,20 @@ def main():
     cs_per_step = sc.channel_samples
     flops_per_step = flops_per_epoch(sc) * sc.epochs
 
-    def batched_step():
-        job.agc.gain, job.agc.clipped, job.agc.components, job.agc.adjustments = 1.0, 0, 0, 0
-        run_batch([job], args.workgroups_per_cu, stream)
-
     for _ in range(args.warmup):
-        batched_step()
-    # Kernel time from HIP events on the launch stream, beside the host wall clock of the whole step.
-    L = _lib.lib()
-    t_wall = []
-    t_kernel = []
-    for _ in range(args.steps):
-        e0, e1 = Event(), Event()
-        L.pharos_synth_device_sync(0)
-        t0 = time.perf_counter()
-        job.agc.gain, job.agc.clipped, job.agc.components, job.agc.adjustments = 1.0, 0, 0, 0
-        arr = (_lib.SynthJob * 1)(job.record())
-        tuning = _lib.SynthTuning(args.workgroups_per_cu, (C.c_uint32 * 3)())
-        h = C.c_void_p()
-        e0.record(stream)
-        _lib.check(L.pharos_synth_batch_submit(arr, 1, C.byref(tuning), stream.handle, C.byref(h)))
-        e1.record(stream)
-        _lib.check(L.pharos_synth_batch_wait(h))
-        L.pharos_synth_device_sync(0)
-        t_wall.append(time.perf_counter() - t0)
-        t_kernel.append(e0.elapsed_ms(e1) / 1e3)
-    ms_per_step = 1e3 * sum(t_wall) / len(t_wall)
-    kernel_s = sum(t_kernel) / len(t_kernel)
+        timed_batch([job], args.workgroups_per_cu, stream)
+    runs = [timed_batch([job], args.workgroups_per_cu, stream) for _ in range(args.steps)]
+    ms_per_step = 1e3 * sum(w for w, _ in runs) / len(runs)
+    kernel_s = sum(k for _, k in runs) / len(runs)
     value = cs_per_step / (ms_per_step / 1e3)
 
-    # Plain form: per-window launches, host AGC between windows (a sync per window).
-    eng = job.engine
-    plain_t = []
-    for _ in range(args.plain_steps):
-        L.pharos_synth_device_sync(0)
-        t0 = time.perf_counter()
-        gain, acc_c, acc_n = 1.0, 0, 0
-        for e in range(0, sc.epochs, 100):
-            n = min(100, sc.epochs - e)
-            c = eng.window(e, n, job.rows.at(e * nch * 64), job.masks.ptr, job.mask_len, float(np.float32(gain)),
-                           job.out.at(e * eng.bytes_per_epoch), clips=True, stream=stream)
-            acc_c += int(c.sum())
-            acc_n += n * sc.samples_per_epoch * 2
-            if (e + n) % 100 == 0:
-                rate = acc_c / acc_n
-                acc_c = acc_n = 0
-                if rate > 0.01:
-                    gain *= 0.95
-                elif rate < 0.001 and gain < 1.0:
-                    gain = min(gain * 1.02, 1.0)
-        L.pharos_synth_device_sync(0)
-        plain_t.append(time.perf_counter() - t0)
+    graph_ms = None
+    if args.graph_steps:
+        g = Graph(job)
+        timed_graphs([g], stream)
+        graph_ms = 1e3 * sum(timed_graphs([g], stream) for _ in range(args.graph_steps)) / args.graph_steps
+        g.close()
+    plain_t = [timed_plain([job], stream) for _ in range(args.plain_steps)]
     plain_value = cs_per_step / (sum(plain_t) / len(plain_t)) if plain_t else None
 
     achieved = flops_per_step / kernel_s / 1e12
@@ -190,6 +183,9 @@ def main():
                      "kernel": "synth_batch_kernel", "kernel_ms": kernel_s * 1e3,
                      "flops_per_launch": flops_per_step, "hbm_write_gbs": out_bytes / kernel_s / 1e9,
                      "hbm_write_frac": out_bytes / kernel_s / 1e9 / PEAK_HBM_GBS},
+        "graph": {"value": cs_per_step / (graph_ms / 1e3) if graph_ms else None, "unit": "channel-samples/s",
+                  "ms_per_step": graph_ms,
+                  "form": "per-window launches with the AGC step on the device, one HIP graph launch per run"},
         "plain": {"value": plain_value, "unit": "channel-samples/s",
                   "ms_per_step": 1e3 * sum(plain_t) / len(plain_t) if plain_t else None,
                   "form": "per-window launches (100 epochs), host AGC between windows"},
@@ -198,9 +194,97 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    return line
+
+
+def bench_c3(args):
+    from concurrent.futures import ThreadPoolExecutor
+    from pharos_amd import scenario as S
+    from pharos_amd.engine import Job, Graph, Stream
+    cap = args.c3_epochs_cap or None
+    scs = S.c3(seed=1, epochs_cap=cap)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as pool:
+        inputs = list(pool.map(lambda s: s.window(0, s.epochs), scs))
+    jobs = [Job(s, rows=r, masks=m) for s, (r, m) in zip(scs, inputs)]
+    del inputs
+    stream = Stream()
+    cs = sum(s.channel_samples for s in scs)
+    flops = sum(flops_per_epoch(s) * s.epochs for s in scs)
+    timed_batch(jobs, args.workgroups_per_cu, stream)
+    runs = [timed_batch(jobs, args.workgroups_per_cu, stream) for _ in range(args.steps)]
+    wall = sum(w for w, _ in runs) / len(runs)
+    kern = sum(k for _, k in runs) / len(runs)
+    graphs = [Graph(j) for j in jobs] if args.graph_steps else []
+    graph_s = None
+    if graphs:
+        timed_graphs(graphs, stream)
+        graph_s = sum(timed_graphs(graphs, stream) for _ in range(args.graph_steps)) / args.graph_steps
+    plain_s = timed_plain(jobs, stream) if args.plain_steps else None
+    fmts = {}
+    for s in scs:
+        fmts[s.format] = fmts.get(s.format, 0) + 1
+    return {
+        "metric": METRIC, "value": cs / wall, "unit": "channel-samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (256 seeded scenarios; rows, masks and tables resident in HBM)",
+        "config": {"workload": "uneven_256_scenarios", "scenarios": len(scs), "epochs_cap": cap,
+                   "channel_samples_per_step": cs, "output_bytes": sum(s.epochs * s.bytes_per_epoch for s in scs),
+                   "formats": fmts, "form": "one batched persistent launch per output format"},
+        "roofline": {"bound": "valu", "achieved": flops / kern / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": flops / kern / 1e12 / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "synth_batch_kernel",
+                     "kernel_ms": kern * 1e3},
+        "graph": {"value": cs / graph_s if graph_s else None, "ms_per_step": graph_s * 1e3 if graph_s else None,
+                  "form": "one HIP graph per scenario (per-window launches, device AGC), replayed in sequence"},
+        "plain": {"value": cs / plain_s if plain_s else None, "ms_per_step": plain_s * 1e3 if plain_s else None,
+                  "form": "per-scenario per-window launches with the host AGC"},
+        "batched_over_plain": (plain_s / wall) if plain_s else None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["c1", "c3"], default="c1",
+                    help="c1: the multi-GNSS 40-channel run (default); c3: the uneven 256-scenario batch")
+    ap.add_argument("--epochs", type=int, default=60_000, help="epochs per run (the configuration has 60 000)")
+    ap.add_argument("--c3-epochs-cap", type=int, default=3000, help="longest scenario of the uneven batch (0: none)")
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--graph-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve", action="store_true", help="also run the Poisson serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="requests per second")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window")
+    ap.add_argument("--serve-epochs-cap", type=int, default=0, help="longest request in epochs (0: up to 30 s)")
+    ap.add_argument("--workgroups-per-cu", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device three times (a known-size reference for HBM counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (runs shard by scenario or epoch "
+              f"range with no exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    from pharos_amd.engine import DeviceBuffer, require_gpu
+    from pharos_amd import _lib
+    require_gpu()
+    if args.copy_calibration:
+        a, b = DeviceBuffer(args.copy_calibration), DeviceBuffer(args.copy_calibration)
+        for _ in range(3):
+            _lib.check(_lib.lib().pharos_synth_memcpy(b.ptr, a.ptr, args.copy_calibration, 2))
+        _lib.lib().pharos_synth_device_sync(0)
+        print(json.dumps({"copied_bytes": args.copy_calibration, "copies": 3}), flush=True)
+        return
+
+    line = bench_c3(args) if args.workload == "c3" else bench_c1(args)
     if args.serve:
         from pharos_amd import serve
-        line["serving"] = serve.measure(duration_s=20.0)
+        line["serving"] = serve.measure(duration_s=args.serve_seconds, rate_hz=args.serve_rate,
+                                        epochs_cap=args.serve_epochs_cap or None)
+        if "cpu_baseline" in line:
+            line["serving"]["over_cpu"] = line["serving"]["channel_samples_per_s"] / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 
 

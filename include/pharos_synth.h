@@ -157,6 +157,17 @@ int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t job_count, 
 int pharos_synth_batch_query(pharos_synth_batch_handle *handle); /* 1 done, 0 running, <0 error */
 int pharos_synth_batch_wait(pharos_synth_batch_handle *handle);  /* frees the handle */
 
+/* Graph form: one job's windows as per-window launches with the AGC step between them on the device, captured once
+ * into a HIP graph and replayed with no host round trip (the window loop of engine/run.zig:225-256 as one graph
+ * launch).  The job's buffers and host structs must outlive the graph.  Each launch starts from the job's AGC state
+ * as it is at launch time; finish waits for the stream and writes the AGC state, gain changes and clip total back
+ * into the job, like pharos_synth_batch_wait.  A replay's bytes equal the plain and batched forms'. */
+typedef struct pharos_synth_graph pharos_synth_graph;
+int pharos_synth_graph_create(const pharos_synth_job *job, pharos_synth_graph **out);
+int pharos_synth_graph_launch(pharos_synth_graph *graph, void *stream);
+int pharos_synth_graph_finish(pharos_synth_graph *graph, void *stream);
+uint32_t pharos_synth_graph_windows(const pharos_synth_graph *graph);
+void pharos_synth_graph_destroy(pharos_synth_graph *graph);
 
 /* Device plumbing for callers without a HIP runtime of their own (the library's runtime owns the device).
  * Copies are synchronous; kind 0 host to device, 1 device to host, 2 device to device. */

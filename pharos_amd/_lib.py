@@ -106,6 +106,11 @@ def lib():
         "pharos_synth_batch_submit": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP, P(VP)]),
         "pharos_synth_batch_query": (C.c_int, [VP]),
         "pharos_synth_batch_wait": (C.c_int, [VP]),
+        "pharos_synth_graph_create": (C.c_int, [P(SynthJob), P(VP)]),
+        "pharos_synth_graph_launch": (C.c_int, [VP, VP]),
+        "pharos_synth_graph_finish": (C.c_int, [VP, VP]),
+        "pharos_synth_graph_windows": (U32, [VP]),
+        "pharos_synth_graph_destroy": (None, [VP]),
         "pharos_synth_device_count": (C.c_int, []),
         "pharos_synth_mem_info": (C.c_int, [C.c_int, P(U64), P(U64)]),
         "pharos_synth_malloc": (C.c_int, [C.c_int, U64, P(VP)]),

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngin
                                                                uint32_t channels, const uint8_t *__restrict__ masks,
                                                                uint64_t mask_len, uint64_t first_epoch, float gain,
                                                                uint8_t *__restrict__ out, uint32_t *__restrict__ clip_partial,
-                                                               uint32_t *err) {
+                                                               uint32_t *err, const float *gain_dev) {
     extern __shared__ __align__(16) uint8_t smem[];
     const uint32_t tile = blockIdx.x, e = blockIdx.y;
     TileArgs A;
@@ -70,7 +70,8 @@ __global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngin
     A.tile = tile;
     A.epoch = first_epoch + e;
     A.out_epoch = out + (size_t)e * E->bpe;
-    uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain; }, []() {}, err);
+    // The gain is the argument, or a device word another kernel of the same stream wrote (the graph form).
+    uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain_dev ? *gmem(gain_dev) : gain; }, []() {}, err);
     if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
 }
 
@@ -108,14 +109,12 @@ __device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hi
     return lo;
 }
 
-// One AGC step after a window (gain.zig:19-35, accounted as engine/run.zig:240-246 does per window).
-__device__ void agc_account(const BatchState &S, uint64_t first_epoch, uint32_t spe, uint32_t job, const DevSegment &seg,
-                            uint32_t s) {
-    DevAgc &G = S.agc[job];
-    unsigned long long clipped = __hip_atomic_load(&S.seg_clip[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Agc.account over one window (gain.zig:19-35, called per window as engine/run.zig:240-246 does): the counts
+// accumulate, and at an evaluation epoch (a multiple of 100) the clip rate moves the gain.
+__device__ void agc_step(DevAgc &G, unsigned long long clipped, unsigned long long components,
+                         unsigned long long epoch_after, DevAdjust *log, uint32_t cap) {
     G.clipped += clipped;
-    G.components += (unsigned long long)seg.epochs * spe * 2ull;
-    const unsigned long long epoch_after = first_epoch + seg.epoch_off + seg.epochs;
+    G.components += components;
     if (epoch_after % 100ull == 0) {
         const double rate = (double)G.clipped / (double)G.components;
         G.clipped = 0;
@@ -124,8 +123,8 @@ __device__ void agc_account(const BatchState &S, uint64_t first_epoch, uint32_t 
         if (rate > 0.01) { G.gain *= 0.95; changed = 1; reduced = 1; }
         else if (rate < 0.001 && G.gain < 1.0) { double g = G.gain * 1.02; G.gain = g < 1.0 ? g : 1.0; changed = 1; }
         if (changed) {
-            if (G.adjustments < S.adjust_cap) {
-                DevAdjust &a = S.adjust[(size_t)job * S.adjust_cap + G.adjustments];
+            if (G.adjustments < cap) {
+                DevAdjust &a = log[G.adjustments];
                 a.epoch = epoch_after;
                 a.gain = G.gain;
                 a.rate = rate;
@@ -134,10 +133,48 @@ __device__ void agc_account(const BatchState &S, uint64_t first_epoch, uint32_t 
             G.adjustments++;
         }
     }
+}
+
+// The AGC step of a finished segment, run by its last tile; publishes the next segment's gain.
+__device__ void agc_account(const BatchState &S, uint64_t first_epoch, uint32_t spe, uint32_t job, const DevSegment &seg,
+                            uint32_t s) {
+    unsigned long long clipped = __hip_atomic_load(&S.seg_clip[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    DevAgc &G = S.agc[job];
+    agc_step(G, clipped, (unsigned long long)seg.epochs * spe * 2ull, first_epoch + seg.epoch_off + seg.epochs,
+             S.adjust + (size_t)job * S.adjust_cap, S.adjust_cap);
     const uint32_t nxt = S.seg_next[s];
     if (nxt != 0xffffffffu) {
         S.seg_gain[nxt] = (float)G.gain;
         __hip_atomic_store(&S.seg_ready[nxt], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Graph form: the AGC state enters from `initial` and the first window's gain is published to gain_dev.
+__global__ void agc_begin_kernel(DevAgc *G, const DevAgc *initial, float *gain_dev) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *G = *initial;
+        *gain_dev = (float)G->gain;
+    }
+}
+
+// Graph form: one window's AGC step from its tiles' clip counts (one block of AGC_THREADS), then the next gain.
+constexpr uint32_t AGC_THREADS = 256;
+__global__ __launch_bounds__(AGC_THREADS) void agc_window_kernel(DevAgc *G, const uint32_t *clip_partial, uint32_t count,
+                                                                  unsigned long long components,
+                                                                  unsigned long long epoch_after, float *gain_dev,
+                                                                  DevAdjust *log, uint32_t cap) {
+    __shared__ unsigned long long part[AGC_THREADS];
+    unsigned long long sum = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += AGC_THREADS) sum += clip_partial[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t h = AGC_THREADS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        agc_step(*G, part[0], components, epoch_after, log, cap);
+        *gain_dev = (float)G->gain;
     }
 }
 
@@ -459,7 +496,7 @@ extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch,
 #define PHAROS_WINDOW(FMT)                                                                                         \
         case FMT:                                                                                                \
             hipLaunchKernelGGL(synth_window_kernel<FMT>, grid, dim3(THREADS), lds, stream, e->d_engine, r, channels, \
-                               masks, mask_len, first_epoch + done, gain, o, cp, e->d_err);                     \
+                               masks, mask_len, first_epoch + done, gain, o, cp, e->d_err, nullptr);            \
             break;
         PHAROS_WINDOW(PHAROS_SYNTH_IQ2)
         PHAROS_WINDOW(PHAROS_SYNTH_IQ4)
@@ -804,6 +841,182 @@ extern "C" int pharos_synth_batch(const pharos_synth_job *jobs, uint32_t njobs, 
     if (st) return st;
     return pharos_synth_batch_wait(h);
 }
+
+// ------------------------------------------------------------------ graph form
+// One run's windows as plain launches with the AGC step between them on the device, captured once into a HIP graph
+// and replayed with no host round trip: the reference's window loop (engine/run.zig:225-256) as one graph launch.
+struct pharos_synth_graph {
+    int device = 0;
+    pharos_synth_engine *engine = nullptr;
+    pharos_synth_job job{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *mem = nullptr;
+    DevAgc *d_agc = nullptr, *d_initial = nullptr;
+    float *d_gain = nullptr;
+    DevAdjust *d_adjust = nullptr;
+    uint32_t *d_clip = nullptr;
+    size_t nclip = 0;
+    uint32_t cap = 1, windows = 0;
+    DevAgc initial{};
+};
+
+static void graph_free(pharos_synth_graph *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->mem) (void)hipFree(g->mem);
+    delete g;
+}
+
+static void launch_window(pharos_synth_engine *e, const dim3 &grid, hipStream_t stream, const pharos_synth_row *rows,
+                          uint32_t channels, const uint8_t *masks, uint64_t mask_len, uint64_t first_epoch, float gain,
+                          uint8_t *out, uint32_t *clip, const float *gain_dev) {
+    const uint32_t lds = e->view.lds_bytes;
+    switch (e->view.format) {
+#define PHAROS_GRAPH_WINDOW(FMT)                                                                                   \
+    case FMT:                                                                                                      \
+        hipLaunchKernelGGL(synth_window_kernel<FMT>, grid, dim3(THREADS), lds, stream, e->d_engine, rows, channels, \
+                           masks, mask_len, first_epoch, gain, out, clip, e->d_err, gain_dev);                      \
+        break;
+    PHAROS_GRAPH_WINDOW(PHAROS_SYNTH_IQ2)
+    PHAROS_GRAPH_WINDOW(PHAROS_SYNTH_IQ4)
+    PHAROS_GRAPH_WINDOW(PHAROS_SYNTH_IQ8)
+    PHAROS_GRAPH_WINDOW(PHAROS_SYNTH_IQ16)
+    PHAROS_GRAPH_WINDOW(PHAROS_SYNTH_F32)
+#undef PHAROS_GRAPH_WINDOW
+    }
+}
+
+extern "C" int pharos_synth_graph_create(const pharos_synth_job *job, pharos_synth_graph **out) {
+    g_error[0] = 0;
+    if (!job || !out) { set_error("null argument"); return ST_ARG; }
+    *out = nullptr;
+    const pharos_synth_job &J = *job;
+    if (!J.engine || !J.out || (J.channels && !J.rows) || J.epochs == 0) { set_error("the job is incomplete"); return ST_ARG; }
+    if (J.gain_mode > 1) { set_error("gain mode %u is not defined", J.gain_mode); return ST_ARG; }
+    if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT && !(J.constant_gain > 0.0f)) { set_error("constant gain must be positive"); return ST_ARG; }
+    if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE && !J.agc) { set_error("adaptive gain needs its AGC state"); return ST_ARG; }
+    pharos_synth_engine *e = J.engine;
+    HIP_TRY(hipSetDevice(e->device), ST_INTERNAL);
+    pharos_synth_graph *g = new (std::nothrow) pharos_synth_graph();
+    if (!g) { set_error("out of memory"); return ST_NOMEM; }
+    g->device = e->device;
+    g->engine = e;
+    g->job = J;
+    g->cap = std::max<uint32_t>(1, J.adjustments_cap);
+    const uint32_t tiles = e->view.tiles;
+    g->nclip = (size_t)J.epochs * tiles;
+    size_t off_agc = 0, off_init = 256, off_gain = 512, off_adj = 768;
+    size_t off_clip = (off_adj + sizeof(DevAdjust) * g->cap + 255) & ~(size_t)255;
+    size_t total = off_clip + g->nclip * sizeof(uint32_t);
+    if (hipMalloc(&g->mem, total) != hipSuccess) { set_error("hipMalloc(%zu) failed", total); graph_free(g); return ST_NOMEM; }
+    uint8_t *b = (uint8_t *)g->mem;
+    g->d_agc = (DevAgc *)(b + off_agc);
+    g->d_initial = (DevAgc *)(b + off_init);
+    g->d_gain = (float *)(b + off_gain);
+    g->d_adjust = (DevAdjust *)(b + off_adj);
+    g->d_clip = (uint32_t *)(b + off_clip);
+
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) { set_error("stream create failed"); graph_free(g); return ST_INTERNAL; }
+    const bool adaptive = J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE;
+    const uint32_t bpe = (uint32_t)e->view.bpe;
+    hipError_t st = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (st == hipSuccess) {
+        if (adaptive) hipLaunchKernelGGL(agc_begin_kernel, dim3(1), dim3(64), 0, cs, g->d_agc, g->d_initial, g->d_gain);
+        uint32_t off = 0;
+        while (off < J.epochs) {
+            const uint64_t abs_e = J.first_epoch + off;
+            const uint32_t n = std::min<uint32_t>((uint32_t)(100 - abs_e % 100), J.epochs - off);
+            for (uint32_t done = 0; done < n;) {
+                const uint32_t m = std::min<uint32_t>(n - done, 65535u);
+                launch_window(e, dim3(tiles, m), cs, J.rows + (size_t)(off + done) * J.channels, J.channels, J.masks,
+                              J.mask_len, abs_e + done, adaptive ? 1.0f : J.constant_gain,
+                              J.out + (size_t)(off + done) * bpe, g->d_clip + (size_t)(off + done) * tiles,
+                              adaptive ? g->d_gain : nullptr);
+                done += m;
+            }
+            if (adaptive)
+                hipLaunchKernelGGL(agc_window_kernel, dim3(1), dim3(AGC_THREADS), 0, cs, g->d_agc,
+                                   g->d_clip + (size_t)off * tiles, n * tiles,
+                                   (unsigned long long)n * e->view.spe * 2ull, (unsigned long long)(abs_e + n),
+                                   g->d_gain, g->d_adjust, g->cap);
+            off += n;
+            g->windows++;
+        }
+        st = hipStreamEndCapture(cs, &g->graph);
+    }
+    if (st == hipSuccess) st = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (st != hipSuccess) {
+        set_error("graph capture failed: %s", hipGetErrorString(st));
+        graph_free(g);
+        return ST_INTERNAL;
+    }
+    *out = g;
+    return ST_OK;
+}
+
+// Replays the run on `stream`, starting from the job's AGC state as it is now (host).
+extern "C" int pharos_synth_graph_launch(pharos_synth_graph *g, void *stream_) {
+    g_error[0] = 0;
+    if (!g) { set_error("null graph"); return ST_ARG; }
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(g->device), ST_INTERNAL);
+    if (g->job.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) {
+        const pharos_synth_agc &a = *g->job.agc;
+        g->initial.gain = a.gain;
+        g->initial.clipped = a.clipped;
+        g->initial.components = a.components;
+        g->initial.adjustments = 0;
+        g->initial.pad = 0;
+        HIP_TRY(hipMemcpyAsync(g->d_initial, &g->initial, sizeof(DevAgc), hipMemcpyHostToDevice, stream), ST_INTERNAL);
+    }
+    HIP_TRY(hipGraphLaunch(g->exec, stream), ST_INTERNAL);
+    return ST_OK;
+}
+
+// Waits for the replay on `stream` and hands back the job's AGC state, gain changes and clip total.
+extern "C" int pharos_synth_graph_finish(pharos_synth_graph *g, void *stream_) {
+    g_error[0] = 0;
+    if (!g) { set_error("null graph"); return ST_ARG; }
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(g->device), ST_INTERNAL);
+    HIP_TRY(hipStreamSynchronize(stream), ST_INTERNAL);
+    std::vector<uint32_t> clip(g->nclip);
+    HIP_TRY(hipMemcpy(clip.data(), g->d_clip, g->nclip * sizeof(uint32_t), hipMemcpyDeviceToHost), ST_INTERNAL);
+    uint64_t total = 0;
+    for (uint32_t c : clip) total += c;
+    const pharos_synth_job &J = g->job;
+    if (J.clipped) *J.clipped = total;
+    if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) {
+        DevAgc a;
+        HIP_TRY(hipMemcpy(&a, g->d_agc, sizeof a, hipMemcpyDeviceToHost), ST_INTERNAL);
+        const uint32_t prior = J.agc->adjustments;
+        J.agc->gain = a.gain;
+        J.agc->clipped = a.clipped;
+        J.agc->components = a.components;
+        J.agc->adjustments = prior + a.adjustments;
+        if (J.adjustments && a.adjustments) {
+            const uint32_t n = std::min(a.adjustments, std::min(J.adjustments_cap, g->cap));
+            std::vector<DevAdjust> adj(n);
+            HIP_TRY(hipMemcpy(adj.data(), g->d_adjust, n * sizeof(DevAdjust), hipMemcpyDeviceToHost), ST_INTERNAL);
+            for (uint32_t k = 0; k < n; k++) {
+                J.adjustments[k].epoch = adj[k].epoch;
+                J.adjustments[k].gain = adj[k].gain;
+                J.adjustments[k].clip_rate = adj[k].rate;
+                J.adjustments[k].reduced = adj[k].reduced;
+                J.adjustments[k].reserved = 0;
+            }
+        }
+    }
+    return take_device_error(g->engine, stream);
+}
+
+extern "C" uint32_t pharos_synth_graph_windows(const pharos_synth_graph *g) { return g ? g->windows : 0; }
+extern "C" void pharos_synth_graph_destroy(pharos_synth_graph *g) { graph_free(g); }
 
 // ------------------------------------------------------------------ device plumbing
 extern "C" int pharos_synth_device_count(void) {

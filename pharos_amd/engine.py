@@ -219,9 +219,79 @@ class Job:
         return self.out.to_numpy()
 
 
+def plain_job(job, stream=None, window=100):
+    """The plain form over a Job's device buffers: per-window launches with the host AGC between windows (the
+    reference's window loop, engine/run.zig:225-256 with gain.zig:19-35).  Returns (clip total, final gain,
+    adjustments as (epoch, gain, rate))."""
+    sc, eng = job.scenario, job.engine
+    nch = len(sc.channels)
+    cg = job.constant_gain
+    gain, acc_c, acc_n, total, adjustments = 1.0, 0, 0, 0, []
+    e = 0
+    while e < job.epochs:
+        absolute = job.first_epoch + e
+        n = min(window - absolute % window, job.epochs - e)
+        gw = cg if cg is not None else float(np.float32(gain))
+        c = eng.window(absolute, n, job.rows.at(e * nch * 64), job.masks.ptr, job.mask_len, gw,
+                       job.out.at(e * eng.bytes_per_epoch), clips=True, stream=stream)
+        total += int(c.sum())
+        if cg is None:
+            acc_c += int(c.sum())
+            acc_n += n * sc.samples_per_epoch * 2
+            if (absolute + n) % 100 == 0:
+                rate = acc_c / acc_n
+                acc_c = acc_n = 0
+                if rate > 0.01:
+                    gain *= 0.95
+                    adjustments.append((absolute + n, gain, rate))
+                elif rate < 0.001 and gain < 1.0:
+                    gain = min(gain * 1.02, 1.0)
+                    adjustments.append((absolute + n, gain, rate))
+        e += n
+    return total, gain, adjustments
+
+
 def run_batch(jobs, workgroups_per_cu=0, stream=None):
     """Runs jobs (a list of Job) in one persistent launch and waits for it."""
     arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
     tuning = _lib.SynthTuning(workgroups_per_cu, (C.c_uint32 * 3)())
     check(_lib.lib().pharos_synth_batch(arr, len(jobs), C.byref(tuning), stream.handle if stream else None))
     return jobs
+
+
+class Graph:
+    """A job's window loop captured as a HIP graph (pharos_synth_graph_*): AGC steps run on the device between
+    windows, and a launch replays the whole run."""
+
+    def __init__(self, job):
+        self.job = job
+        self._record = job.record()  # kept alive: the graph holds its host pointers
+        h = C.c_void_p()
+        check(_lib.lib().pharos_synth_graph_create(C.byref(self._record), C.byref(h)))
+        self.handle = h.value
+
+    @property
+    def windows(self):
+        return int(_lib.lib().pharos_synth_graph_windows(self.handle))
+
+    def launch(self, stream=None):
+        check(_lib.lib().pharos_synth_graph_launch(self.handle, stream.handle if stream else None))
+
+    def finish(self, stream=None):
+        check(_lib.lib().pharos_synth_graph_finish(self.handle, stream.handle if stream else None))
+
+    def run(self, stream=None):
+        self.launch(stream)
+        self.finish(stream)
+        return self.job
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().pharos_synth_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,213 @@
+"""Continuous-batching request loop over the batched engine (one GPU).
+
+Requests are runs (scenarios) arriving over time. The loop works in iterations:
+1. It admits every arrived request whose device memory fits the free HBM (pharos_synth_mem_info), oldest first.
+2. It submits one persistent batch holding the next slice (SLICE epochs, a multiple of the 100-epoch AGC window)
+   of every admitted request.
+3. While the GPU runs that batch, the host derives and uploads each request's following slice.
+4. It retires the requests that completed.
+
+A request's AGC state (pharos_synth_agc) carries from slice to slice, so its bytes are those of the same run alone,
+whatever else shares its batches. Latency is measured from a request's arrival to the end of the batch that
+writes its last epoch. Its output is in device memory when `on_complete` is called with it, and is freed after
+that unless the loop is asked to keep outputs.
+"""
+import collections
+import ctypes as C
+import os
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+from .engine import Engine, DeviceBuffer, Stream, require_gpu
+
+SLICE = 1000            # epochs per request per iteration (ten AGC windows)
+RESERVE = 4 << 30       # HBM left free for the library's own allocations
+
+
+class Request:
+    def __init__(self, scenario, arrival_s=0.0, rid=0):
+        self.scenario = scenario
+        self.arrival = float(arrival_s)
+        self.id = rid
+        self.engine = None
+        self.out = None
+        self.done_epochs = 0
+        self.agc = _lib.SynthAgc(1.0, 0, 0, 0, 0)
+        self.adjust = (_lib.SynthAdjustment * 4096)()
+        self.clipped_total = 0
+        self.slice_clip = C.c_uint64(0)
+        self.staged = None       # (first_epoch, epochs, DeviceBuffer rows, DeviceBuffer masks, mask_len)
+        self.admitted = None
+        self.completed = None
+
+    @property
+    def epochs(self):
+        return self.scenario.epochs
+
+    @property
+    def out_bytes(self):
+        return self.scenario.epochs * self.scenario.bytes_per_epoch
+
+    def slice_bytes(self):
+        n = min(SLICE, self.epochs)
+        return n * max(1, len(self.scenario.channels)) * 64 * 2 + 16 * n * max(1, len(self.scenario.channels))
+
+    def device_need(self):
+        return self.out_bytes + 2 * self.slice_bytes() + (8 << 20)
+
+    def gain_mode(self):
+        return 1 if self.scenario.gain_db is not None else 0
+
+    def constant_gain(self):
+        g = self.scenario.gain_db
+        return 0.0 if g is None else float(np.float32(10.0 ** (g / 20.0)))
+
+    def adjustments(self):
+        n = min(self.agc.adjustments, len(self.adjust))
+        return [(int(a.epoch), float(a.gain), float(a.clip_rate)) for a in self.adjust[:n]]
+
+
+def _prepare(req, first):
+    """Host rows and masks of the slice starting at `first` (runs in a worker thread; the C stage releases the GIL)."""
+    n = min(SLICE, req.epochs - first)
+    rows, masks = req.scenario.window(first, n)
+    return first, n, rows, masks
+
+
+class Server:
+    def __init__(self, device=0, threads=None, reserve=RESERVE, workgroups_per_cu=0):
+        require_gpu()
+        self.device = device
+        self.reserve = reserve
+        self.tuning = _lib.SynthTuning(workgroups_per_cu, (C.c_uint32 * 3)())
+        self.pool = ThreadPoolExecutor(max_workers=threads or min(16, os.cpu_count() or 4))
+        self.stream = Stream(device)
+        self.iterations = 0
+        self.batch_sizes = []
+
+    def _free_hbm(self):
+        free, total = C.c_uint64(), C.c_uint64()
+        check(_lib.lib().pharos_synth_mem_info(self.device, C.byref(free), C.byref(total)))
+        return int(free.value)
+
+    def _admit(self, req):
+        req.engine = Engine(req.scenario, self.device)
+        req.out = DeviceBuffer(max(req.out_bytes, 16), self.device)
+        req.admitted = time.perf_counter()
+
+    def _upload(self, req, prepared):
+        first, n, rows, masks = prepared
+        req.staged = (first, n, DeviceBuffer.from_array(rows, self.device), DeviceBuffer.from_array(masks, self.device),
+                      len(masks))
+
+    def _job(self, req):
+        first, n, d_rows, d_masks, mask_len = req.staged
+        prior = min(req.agc.adjustments, len(req.adjust))
+        adj = C.cast(C.c_void_p(C.addressof(req.adjust) + prior * C.sizeof(_lib.SynthAdjustment)),
+                     C.POINTER(_lib.SynthAdjustment))
+        cap = len(req.adjust) - prior
+        req.slice_clip.value = 0
+        return _lib.SynthJob(req.engine.handle.value, first, n, len(req.scenario.channels), d_rows.ptr, d_masks.ptr,
+                             mask_len, req.out.at(first * req.scenario.bytes_per_epoch), req.gain_mode(),
+                             req.constant_gain(), C.pointer(req.agc), adj, cap, 0, C.pointer(req.slice_clip))
+
+    def serve(self, requests, on_complete=None, keep_outputs=False):
+        """Runs `requests` (arrival times relative to the call) to completion; returns them with .completed set."""
+        L = _lib.lib()
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting, active, finished = collections.deque(), [], []
+        t0 = time.perf_counter()
+        for r in requests:
+            r.arrival_abs = t0 + r.arrival
+        while pending or waiting or active:
+            now = time.perf_counter()
+            while pending and pending[0].arrival_abs <= now:
+                waiting.append(pending.popleft())
+            # Admission by free device memory, oldest first.
+            admitted = []
+            while waiting:
+                need = waiting[0].device_need()
+                if self._free_hbm() - self.reserve < need and active:
+                    break
+                req = waiting.popleft()
+                self._admit(req)
+                admitted.append(req)
+            if admitted:
+                for req, prep in zip(admitted, self.pool.map(lambda q: _prepare(q, 0), admitted)):
+                    self._upload(req, prep)
+                active.extend(admitted)
+            if not active:
+                if pending:
+                    time.sleep(max(0.0, min(0.002, pending[0].arrival_abs - time.perf_counter())))
+                continue
+            # One persistent batch over every active request's staged slice.
+            jobs = (_lib.SynthJob * len(active))(*[self._job(r) for r in active])
+            h = C.c_void_p()
+            check(L.pharos_synth_batch_submit(jobs, len(active), C.byref(self.tuning), self.stream.handle, C.byref(h)))
+            self.iterations += 1
+            self.batch_sizes.append(len(active))
+            # While it runs: the next slice of every request that has one.
+            nxt = [(r, r.staged[0] + r.staged[1]) for r in active if r.staged[0] + r.staged[1] < r.epochs]
+            futures = [(r, self.pool.submit(_prepare, r, f)) for r, f in nxt]
+            prepared = [(r, f.result()) for r, f in futures]
+            check(L.pharos_synth_batch_wait(h))
+            end = time.perf_counter()
+            still = []
+            for r in active:
+                r.clipped_total += int(r.slice_clip.value)
+                r.done_epochs = r.staged[0] + r.staged[1]
+                if r.done_epochs >= r.epochs:
+                    r.completed = end
+                    r.staged = None
+                    finished.append(r)
+                    if on_complete:
+                        on_complete(r)
+                    if not keep_outputs:
+                        r.out.free()
+                        r.engine.close()
+                else:
+                    still.append(r)
+            for r, prep in prepared:
+                self._upload(r, prep)
+            active = still
+        return finished
+
+
+def make_requests(rate_hz, duration_s, seed=1, max_out_bytes=2 << 30, epochs_cap=None):
+    """Poisson arrivals at `rate_hz` over `duration_s`, each a run drawn as the uneven batch draws them (4-48
+    channels, 4.092-61.44 Msps, 1-30 s, IQ2-IQ16), its length cut so its output stays within max_out_bytes."""
+    from . import scenario as S
+    rng = np.random.default_rng([seed, 4])
+    t, times = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate_hz)
+        if t >= duration_s:
+            break
+        times.append(t)
+    scs = S.c3(seed=seed + 1000, count=len(times), epochs_cap=epochs_cap)
+    reqs = []
+    for i, (at, sc) in enumerate(zip(times, scs)):
+        fit = max(100, (max_out_bytes // sc.bytes_per_epoch) // 100 * 100)
+        sc.epochs = int(min(sc.epochs, fit))
+        reqs.append(Request(sc, at, i))
+    return reqs
+
+
+def measure(duration_s=10.0, rate_hz=20.0, seed=1, epochs_cap=None):
+    """p50/p99 latency and aggregate channel-samples/s of a Poisson request stream."""
+    reqs = make_requests(rate_hz, duration_s, seed, epochs_cap=epochs_cap)
+    server = Server()
+    t0 = time.perf_counter()
+    done = server.serve(reqs)
+    makespan = time.perf_counter() - t0
+    lat = np.array([r.completed - r.arrival_abs for r in done]) * 1e3
+    cs = sum(r.scenario.channel_samples for r in done)
+    return {"requests": len(done), "rate_hz": rate_hz, "arrival_window_s": duration_s, "makespan_s": makespan,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "channel_samples": int(cs), "channel_samples_per_s": cs / makespan,
+            "iterations": server.iterations, "mean_batch_jobs": float(np.mean(server.batch_sizes)),
+            "slice_epochs": SLICE, "epochs_cap": epochs_cap}
