@@ -61,7 +61,7 @@ def cpu_baseline(scenario, seconds=10.0, threads=None):
     dt = time.perf_counter() - t0
     return {"value": nch * scenario.samples_per_epoch * done / dt, "unit": "channel-samples/s", "cores": threads,
             "kind": "port",
-            "sample": f"first {done} epochs ({done / 1000:.1f} s of signal) of the same 40-channel run, "
+            "sample": f"first {done} epochs ({done / 1000:.1f} s of signal) of the same {nch}-channel run, "
                       f"C restatement of kernel.zig on {threads} threads, constant gain, {dt:.1f} s wall"}
 
 
@@ -83,8 +83,9 @@ def _reset(job):
     job.clipped.value = 0
 
 
-def timed_batch(jobs, workgroups_per_cu, stream):
-    """(wall seconds, kernel seconds by HIP events on the launch stream) of one batched launch over `jobs`."""
+def timed_batch(jobs, shape, stream):
+    """(wall seconds, kernel seconds by HIP events on the launch stream) of one batched launch over `jobs` with the
+    launch shape (workgroups per CU, channels per pass)."""
     import ctypes as C
     from pharos_amd import _lib
     from pharos_amd.engine import Event
@@ -92,7 +93,7 @@ def timed_batch(jobs, workgroups_per_cu, stream):
     for j in jobs:
         _reset(j)
     arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
-    tuning = _lib.SynthTuning(workgroups_per_cu, (C.c_uint32 * 3)())
+    tuning = _lib.SynthTuning(shape[0], shape[1], (C.c_uint32 * 2)())
     e0, e1 = Event(), Event()
     h = C.c_void_p()
     L.pharos_synth_device_sync(0)
@@ -132,20 +133,43 @@ def timed_plain(jobs, stream):
     return time.perf_counter() - t0
 
 
-def bench_c1(args):
+def launch_shape(args, scenarios, stream):
+    """The batched launch shape: given on the command line, or chosen by the autotuner on a short sample of the
+    same workload (pharos_amd/tune.py)."""
+    if args.workgroups_per_cu or args.channels_per_pass or args.no_tune:
+        return (args.workgroups_per_cu, args.channels_per_pass), {"source": "command line"}
+    from pharos_amd import tune
+    t = tune.tune(scenarios, sample_epochs=args.tune_epochs, stream=stream)
+    return tuple(t["best"]), {"source": "autotuner", "key": t["key"], "best": {"workgroups_per_cu": t["best"][0],
+                              "channels_per_pass": t["best"][1]}, "sample_epochs": t["sample_epochs"],
+                              "candidates_ms": t["times_ms"], "seconds": t["seconds"]}
+
+
+WORKLOADS = {
+    "c0": ("gps_l1ca_8sat_4.092Msps_iq8", "noise off", "constant 0 dB"),
+    "c1": ("multi_gnss_40ch_61.44Msps_iq4", "counter", "adaptive (device AGC)"),
+    "c2": ("e5_b2_l5_24ch_122.88Msps_iq16", "counter", "adaptive (device AGC)"),
+}
+
+
+def bench_single(args):
+    """One run of the c0, c1 or c2 configuration per step."""
     from pharos_amd import scenario as S
     from pharos_amd.engine import Job, Graph, Stream
-    sc = S.c1(seed=1, epochs=args.epochs)
+    make = {"c0": S.c0, "c1": S.c1, "c2": S.c2}[args.workload]
+    sc = make(seed=1, epochs=args.epochs) if args.epochs else make(seed=1)
+    name, noise, gain = WORKLOADS[args.workload]
     nch = len(sc.channels)
     rows, masks = sc.window(0, sc.epochs)
     job = Job(sc, rows=rows, masks=masks)
     stream = Stream()
     cs_per_step = sc.channel_samples
     flops_per_step = flops_per_epoch(sc) * sc.epochs
+    shape, tuned = launch_shape(args, [sc], stream)
 
     for _ in range(args.warmup):
-        timed_batch([job], args.workgroups_per_cu, stream)
-    runs = [timed_batch([job], args.workgroups_per_cu, stream) for _ in range(args.steps)]
+        timed_batch([job], shape, stream)
+    runs = [timed_batch([job], shape, stream) for _ in range(args.steps)]
     ms_per_step = 1e3 * sum(w for w, _ in runs) / len(runs)
     kernel_s = sum(k for _, k in runs) / len(runs)
     value = cs_per_step / (ms_per_step / 1e3)
@@ -160,7 +184,7 @@ def bench_c1(args):
     plain_value = cs_per_step / (sum(plain_t) / len(plain_t)) if plain_t else None
 
     achieved = flops_per_step / kernel_s / 1e12
-    traffic, valu_busy = profiled(sc.epochs)
+    traffic, valu_busy = profiled(sc.epochs) if args.workload == "c1" else (None, None)
     out_bytes = sc.epochs * sc.bytes_per_epoch
     line = {
         "metric": METRIC,
@@ -175,8 +199,8 @@ def bench_c1(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded channel models; rows, masks and code tables resident in HBM)",
-        "config": {"workload": "multi_gnss_40ch_61.44Msps_iq4", "channels": nch, "sample_rate_hz": 61_440_000,
-                   "format": "IQ4", "epochs": sc.epochs, "noise": "counter", "gain": "adaptive (device AGC)",
+        "config": {"workload": name, "channels": nch, "sample_rate_hz": sc.samples_per_epoch * 1000,
+                   "format": sc.format, "epochs": sc.epochs, "noise": noise, "gain": gain,
                    "form": "batched persistent launch", "channel_samples_per_step": cs_per_step},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "valu_busy_pct_profiled": valu_busy,
@@ -190,6 +214,7 @@ def bench_c1(args):
                   "ms_per_step": 1e3 * sum(plain_t) / len(plain_t) if plain_t else None,
                   "form": "per-window launches (100 epochs), host AGC between windows"},
         "batched_over_plain": value / plain_value if plain_value else None,
+        "tuning": tuned,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
@@ -210,8 +235,9 @@ def bench_c3(args):
     stream = Stream()
     cs = sum(s.channel_samples for s in scs)
     flops = sum(flops_per_epoch(s) * s.epochs for s in scs)
-    timed_batch(jobs, args.workgroups_per_cu, stream)
-    runs = [timed_batch(jobs, args.workgroups_per_cu, stream) for _ in range(args.steps)]
+    shape, tuned = launch_shape(args, scs, stream)
+    timed_batch(jobs, shape, stream)
+    runs = [timed_batch(jobs, shape, stream) for _ in range(args.steps)]
     wall = sum(w for w, _ in runs) / len(runs)
     kern = sum(k for _, k in runs) / len(runs)
     graphs = [Graph(j) for j in jobs] if args.graph_steps else []
@@ -238,6 +264,7 @@ def bench_c3(args):
         "plain": {"value": cs / plain_s if plain_s else None, "ms_per_step": plain_s * 1e3 if plain_s else None,
                   "form": "per-scenario per-window launches with the host AGC"},
         "batched_over_plain": (plain_s / wall) if plain_s else None,
+        "tuning": tuned,
     }
 
 
@@ -246,9 +273,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["c1", "c3"], default="c1",
-                    help="c1: the multi-GNSS 40-channel run (default); c3: the uneven 256-scenario batch")
-    ap.add_argument("--epochs", type=int, default=60_000, help="epochs per run (the configuration has 60 000)")
+    ap.add_argument("--workload", choices=["c0", "c1", "c2", "c3"], default="c1",
+                    help="c1: the multi-GNSS 40-channel run (default); c0: 8 x L1 C/A at 4.092 Msps; c2: the "
+                         "wideband 24-channel 122.88 Msps run; c3: the uneven 256-scenario batch")
+    ap.add_argument("--epochs", type=int, default=0, help="epochs per run (0: the configuration's own)")
     ap.add_argument("--c3-epochs-cap", type=int, default=3000, help="longest scenario of the uneven batch (0: none)")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--graph-steps", type=int, default=2)
@@ -258,7 +286,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window")
     ap.add_argument("--serve-epochs-cap", type=int, default=0, help="longest request in epochs (0: up to 30 s)")
-    ap.add_argument("--workgroups-per-cu", type=int, default=0)
+    ap.add_argument("--workgroups-per-cu", type=int, default=0, help="launch shape (skips the autotuner)")
+    ap.add_argument("--channels-per-pass", type=int, default=0, help="launch shape (skips the autotuner)")
+    ap.add_argument("--no-tune", action="store_true", help="use the library's default launch shape")
+    ap.add_argument("--tune-epochs", type=int, default=300, help="epochs of the autotuner's sample")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device three times (a known-size reference for HBM counters)")
     args = ap.parse_args()
@@ -278,7 +309,7 @@ def main():
         print(json.dumps({"copied_bytes": args.copy_calibration, "copies": 3}), flush=True)
         return
 
-    line = bench_c3(args) if args.workload == "c3" else bench_c1(args)
+    line = bench_c3(args) if args.workload == "c3" else bench_single(args)
     if args.serve:
         from pharos_amd import serve
         line["serving"] = serve.measure(duration_s=args.serve_seconds, rate_hz=args.serve_rate,

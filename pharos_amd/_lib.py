@@ -44,7 +44,7 @@ class SynthJob(C.Structure):
 
 
 class SynthTuning(C.Structure):
-    _fields_ = [("workgroups_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+    _fields_ = [("workgroups_per_cu", C.c_uint32), ("channels_per_pass", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class SignalConstants(C.Structure):

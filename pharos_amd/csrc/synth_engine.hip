@@ -70,6 +70,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngin
     A.tile = tile;
     A.epoch = first_epoch + e;
     A.out_epoch = out + (size_t)e * E->bpe;
+    A.chunk_cap = 0;
     // The gain is the argument, or a device word another kernel of the same stream wrote (the graph form).
     uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain_dev ? *gmem(gain_dev) : gain; }, []() {}, err);
     if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
@@ -91,6 +92,7 @@ struct BatchState {
     uint32_t adjust_cap;
     uint32_t *err;
     uint32_t queue_words_offset;   // LDS byte offset of the work-queue words
+    uint32_t chunk_cap;            // channels per LDS pass at most (0: each engine's own)
 };
 
 __device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
@@ -223,6 +225,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
         A.tile = (uint32_t)(local % seg.tiles);
         A.epoch = J.first_epoch + e_off;
         A.out_epoch = J.out + (size_t)e_off * J.engine->bpe;
+        A.chunk_cap = S.chunk_cap;
         auto gain_fn = [&]() -> float {
             if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT) return J.constant_gain;
             if (threadIdx.x == 0) {
@@ -528,6 +531,7 @@ struct pharos_synth_batch_handle {
     hipStream_t stream = nullptr;
     std::vector<pharos_synth_job> jobs;
     std::vector<void *> owned;
+    std::vector<size_t> owned_bytes;
     DevAgc *d_agc = nullptr;
     DevAdjust *d_adjust = nullptr;
     unsigned long long *d_seg_clip = nullptr;
@@ -538,18 +542,44 @@ struct pharos_synth_batch_handle {
     std::vector<uint8_t> upload;  // host image of the batch tables, alive until the copy has run
 };
 
+// Batch-state blocks outlive their batch in a small per-process cache: a serving loop submits a batch per
+// iteration, and a device allocation and free per batch would cost more than its tables.  A block goes back to the
+// cache only after its batch has finished on the device.
+struct SpareBlock { int device; void *p; size_t n; };
+static std::mutex g_spare_lock;
+static std::vector<SpareBlock> g_spare;
+constexpr size_t SPARE_MAX = 4, SPARE_MIN_BYTES = 1 << 20;
+
 static void batch_free(pharos_synth_batch_handle *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->done) (void)hipEventDestroy(h->done);
-    for (void *p : h->owned) (void)hipFree(p);
+    for (size_t i = 0; i < h->owned.size(); i++) {
+        std::lock_guard<std::mutex> guard(g_spare_lock);
+        if (g_spare.size() < SPARE_MAX) g_spare.push_back({h->device, h->owned[i], h->owned_bytes[i]});
+        else (void)hipFree(h->owned[i]);
+    }
     delete h;
 }
 
 static int batch_alloc(pharos_synth_batch_handle *h, void **p, size_t n) {
-    hipError_t st = hipMalloc(p, n ? n : 16);
-    if (st != hipSuccess) { set_error("hipMalloc(%zu) failed: %s", n, hipGetErrorString(st)); return ST_NOMEM; }
+    {
+        std::lock_guard<std::mutex> guard(g_spare_lock);
+        for (size_t i = 0; i < g_spare.size(); i++) {
+            if (g_spare[i].device == h->device && g_spare[i].n >= n) {
+                *p = g_spare[i].p;
+                h->owned.push_back(*p);
+                h->owned_bytes.push_back(g_spare[i].n);
+                g_spare.erase(g_spare.begin() + (long)i);
+                return ST_OK;
+            }
+        }
+    }
+    const size_t bytes = std::max(n, SPARE_MIN_BYTES);
+    hipError_t st = hipMalloc(p, bytes);
+    if (st != hipSuccess) { set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(st)); return ST_NOMEM; }
     h->owned.push_back(*p);
+    h->owned_bytes.push_back(bytes);
     return ST_OK;
 }
 
@@ -729,6 +759,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.adjust_cap = adjust_cap;
     S.err = (uint32_t *)(b8 + p_err.off);
     S.queue_words_offset = 0;
+    S.chunk_cap = tuning ? std::min<uint32_t>(tuning->channels_per_pass, MAX_CHUNK) : 0;
     h->d_agc = S.agc;
     h->d_adjust = S.adjust;
     h->d_seg_clip = S.seg_clip;

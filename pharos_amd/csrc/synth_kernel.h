@@ -541,6 +541,7 @@ struct TileArgs {
     uint32_t tile;
     uint64_t epoch;                // absolute epoch number
     uint8_t *out_epoch;            // this epoch's slot
+    uint32_t chunk_cap;            // channels per LDS pass at most (0: the engine's)
 };
 
 
@@ -549,7 +550,8 @@ struct TileArgs {
 // persistent kernel waits there for the AGC of the previous window).  Returns the clipped components of the tile.
 template <uint32_t F, class GainFn, class AfterSums>
 __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, AfterSums after_sums, uint32_t *err) {
-    const DevEngine E = load_engine(A.E);
+    DevEngine E = load_engine(A.E);
+    if (A.chunk_cap && A.chunk_cap < E.chunk) E.chunk = A.chunk_cap;
     const uint32_t tid = threadIdx.x;
     const uint32_t b = tid / LANES16, l = tid % LANES16;
     const uint32_t n0 = A.tile * TILE;

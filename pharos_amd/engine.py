@@ -251,10 +251,10 @@ def plain_job(job, stream=None, window=100):
     return total, gain, adjustments
 
 
-def run_batch(jobs, workgroups_per_cu=0, stream=None):
+def run_batch(jobs, workgroups_per_cu=0, stream=None, channels_per_pass=0):
     """Runs jobs (a list of Job) in one persistent launch and waits for it."""
     arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
-    tuning = _lib.SynthTuning(workgroups_per_cu, (C.c_uint32 * 3)())
+    tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, (C.c_uint32 * 2)())
     check(_lib.lib().pharos_synth_batch(arr, len(jobs), C.byref(tuning), stream.handle if stream else None))
     return jobs
 

@@ -2,9 +2,10 @@
 
 Requests are runs (scenarios) arriving over time. The loop works in iterations:
 1. It admits every arrived request whose device memory fits the free HBM (pharos_synth_mem_info), oldest first.
-2. It submits one persistent batch holding the next slice (SLICE epochs, a multiple of the 100-epoch AGC window)
-   of every admitted request.
-3. While the GPU runs that batch, the host derives and uploads each request's following slice.
+2. It submits one persistent batch holding the next slice of every admitted request. A slice is a whole number
+   of 100-epoch AGC windows, sized so it carries about SLICE_WORK channel-samples.
+3. While the GPU runs that batch, the host derives each request's following slice and copies it into the
+   request's second row/mask buffer.
 4. It retires the requests that completed.
 
 A request's AGC state (pharos_synth_agc) carries from slice to slice, so its bytes are those of the same run alone,
@@ -24,7 +25,8 @@ from . import _lib
 from ._lib import check
 from .engine import Engine, DeviceBuffer, Stream, require_gpu
 
-SLICE = 1000            # epochs per request per iteration (ten AGC windows)
+SLICE_MIN = 1000        # epochs of a request per iteration, at least (ten AGC windows) ...
+SLICE_WORK = 8e9        # ... or about this many channel-samples (a few ms of the GPU), in whole AGC windows
 RESERVE = 4 << 30       # HBM left free for the library's own allocations
 
 
@@ -52,12 +54,23 @@ class Request:
     def out_bytes(self):
         return self.scenario.epochs * self.scenario.bytes_per_epoch
 
+    @property
+    def slice_epochs(self):
+        per_epoch = max(1, len(self.scenario.channels)) * self.scenario.samples_per_epoch
+        n = max(SLICE_MIN, int(SLICE_WORK // per_epoch) // 100 * 100)
+        return min(n, self.epochs)
+
     def slice_bytes(self):
-        n = min(SLICE, self.epochs)
-        return n * max(1, len(self.scenario.channels)) * 64 * 2 + 16 * n * max(1, len(self.scenario.channels))
+        """Device bytes of one slice's rows and masks (the mask bound of the scenario stage)."""
+        from . import _lib as L
+        import ctypes as C_
+        s, chans = self.scenario.spec()
+        masks = int(L.lib().pharos_scenario_mask_bound(C_.byref(s), chans, self.slice_epochs)) + 16
+        return self.slice_epochs * max(1, len(self.scenario.channels)) * 64, masks
 
     def device_need(self):
-        return self.out_bytes + 2 * self.slice_bytes() + (8 << 20)
+        rows, masks = self.slice_bytes()
+        return self.out_bytes + 2 * (rows + masks) + (8 << 20)
 
     def gain_mode(self):
         return 1 if self.scenario.gain_db is not None else 0
@@ -72,22 +85,30 @@ class Request:
 
 
 def _prepare(req, first):
-    """Host rows and masks of the slice starting at `first` (runs in a worker thread; the C stage releases the GIL)."""
-    n = min(SLICE, req.epochs - first)
+    """Host rows and masks of the slice starting at `first` (runs in a worker thread; the C stage releases the GIL).
+    A request's first slice is SLICE_MIN epochs, so it starts soon; the later ones carry about SLICE_WORK."""
+    n = min(SLICE_MIN if first == 0 else req.slice_epochs, req.epochs - first)
     rows, masks = req.scenario.window(first, n)
     return first, n, rows, masks
 
 
+def _arrive(req):
+    """Work done as soon as a request arrives, before it is admitted: its code tables and its first slice."""
+    req.scenario.tables()
+    return _prepare(req, 0)
+
+
 class Server:
-    def __init__(self, device=0, threads=None, reserve=RESERVE, workgroups_per_cu=0):
+    def __init__(self, device=0, threads=None, reserve=RESERVE, workgroups_per_cu=0, channels_per_pass=0):
         require_gpu()
         self.device = device
         self.reserve = reserve
-        self.tuning = _lib.SynthTuning(workgroups_per_cu, (C.c_uint32 * 3)())
+        self.tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, (C.c_uint32 * 2)())
         self.pool = ThreadPoolExecutor(max_workers=threads or min(16, os.cpu_count() or 4))
         self.stream = Stream(device)
         self.iterations = 0
         self.batch_sizes = []
+        self.phase_s = collections.Counter()  # host seconds per loop phase
 
     def _free_hbm(self):
         free, total = C.c_uint64(), C.c_uint64()
@@ -97,15 +118,30 @@ class Server:
     def _admit(self, req):
         req.engine = Engine(req.scenario, self.device)
         req.out = DeviceBuffer(max(req.out_bytes, 16), self.device)
+        rows, masks = req.slice_bytes()
+        # Two row/mask buffers per request: the next slice is written into one while the batch reads the other.
+        req.buffers = [(DeviceBuffer(rows, self.device), DeviceBuffer(masks, self.device)) for _ in range(2)]
+        req.flip = 0
         req.admitted = time.perf_counter()
 
     def _upload(self, req, prepared):
         first, n, rows, masks = prepared
-        req.staged = (first, n, DeviceBuffer.from_array(rows, self.device), DeviceBuffer.from_array(masks, self.device),
-                      len(masks))
+        d_rows, d_masks = req.buffers[req.flip]
+        req.flip ^= 1
+        if rows.nbytes > d_rows.nbytes or masks.nbytes > d_masks.nbytes:
+            raise RuntimeError("slice larger than its buffers")
+        if rows.nbytes:
+            check(_lib.lib().pharos_synth_memcpy(d_rows.ptr, rows.ctypes.data, rows.nbytes, 0))
+        check(_lib.lib().pharos_synth_memcpy(d_masks.ptr, masks.ctypes.data, masks.nbytes, 0))
+        req.staged = (first, n, d_rows, d_masks, len(masks))
+
+    def _retire(self, req):
+        req.buffers = None
+        req.staged = None
 
     def _job(self, req):
         first, n, d_rows, d_masks, mask_len = req.staged
+        req.current = (first, n)
         prior = min(req.agc.adjustments, len(req.adjust))
         adj = C.cast(C.c_void_p(C.addressof(req.adjust) + prior * C.sizeof(_lib.SynthAdjustment)),
                      C.POINTER(_lib.SynthAdjustment))
@@ -126,43 +162,57 @@ class Server:
         while pending or waiting or active:
             now = time.perf_counter()
             while pending and pending[0].arrival_abs <= now:
-                waiting.append(pending.popleft())
-            # Admission by free device memory, oldest first.
+                req = pending.popleft()
+                req.arrived = self.pool.submit(_arrive, req)
+                waiting.append(req)
+            # Admission by free device memory, oldest first, of requests whose first slice is ready.
             admitted = []
-            while waiting:
+            ta = time.perf_counter()
+            while waiting and (waiting[0].arrived.done() or not active):
                 need = waiting[0].device_need()
                 if self._free_hbm() - self.reserve < need and active:
                     break
                 req = waiting.popleft()
                 self._admit(req)
                 admitted.append(req)
-            if admitted:
-                for req, prep in zip(admitted, self.pool.map(lambda q: _prepare(q, 0), admitted)):
-                    self._upload(req, prep)
-                active.extend(admitted)
+            tb = time.perf_counter()
+            for req in admitted:
+                self._upload(req, req.arrived.result())
+                req.arrived = None
+            active.extend(admitted)
+            tc = time.perf_counter()
+            self.phase_s["admit"] += tb - ta
+            self.phase_s["first_slice"] += tc - tb
             if not active:
-                if pending:
+                if pending and not waiting:
                     time.sleep(max(0.0, min(0.002, pending[0].arrival_abs - time.perf_counter())))
                 continue
             # One persistent batch over every active request's staged slice.
             jobs = (_lib.SynthJob * len(active))(*[self._job(r) for r in active])
             h = C.c_void_p()
             check(L.pharos_synth_batch_submit(jobs, len(active), C.byref(self.tuning), self.stream.handle, C.byref(h)))
+            td = time.perf_counter()
+            self.phase_s["submit"] += td - tc
             self.iterations += 1
             self.batch_sizes.append(len(active))
-            # While it runs: the next slice of every request that has one.
-            nxt = [(r, r.staged[0] + r.staged[1]) for r in active if r.staged[0] + r.staged[1] < r.epochs]
+            # While it runs: the next slice of every request that has one, derived on the host and copied into the
+            # request's other row/mask buffer.
+            nxt = [(r, sum(r.current)) for r in active if sum(r.current) < r.epochs]
             futures = [(r, self.pool.submit(_prepare, r, f)) for r, f in nxt]
-            prepared = [(r, f.result()) for r, f in futures]
+            for r, f in futures:
+                self._upload(r, f.result())
+            te = time.perf_counter()
             check(L.pharos_synth_batch_wait(h))
             end = time.perf_counter()
+            self.phase_s["next_slices"] += te - td
+            self.phase_s["wait"] += end - te
             still = []
             for r in active:
                 r.clipped_total += int(r.slice_clip.value)
-                r.done_epochs = r.staged[0] + r.staged[1]
+                r.done_epochs = sum(r.current)
                 if r.done_epochs >= r.epochs:
                     r.completed = end
-                    r.staged = None
+                    self._retire(r)
                     finished.append(r)
                     if on_complete:
                         on_complete(r)
@@ -171,8 +221,6 @@ class Server:
                         r.engine.close()
                 else:
                     still.append(r)
-            for r, prep in prepared:
-                self._upload(r, prep)
             active = still
         return finished
 
@@ -210,4 +258,5 @@ def measure(duration_s=10.0, rate_hz=20.0, seed=1, epochs_cap=None):
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "channel_samples": int(cs), "channel_samples_per_s": cs / makespan,
             "iterations": server.iterations, "mean_batch_jobs": float(np.mean(server.batch_sizes)),
-            "slice_epochs": SLICE, "epochs_cap": epochs_cap}
+            "host_phase_s": {k: round(v, 3) for k, v in server.phase_s.items()},
+            "slice": {"min_epochs": SLICE_MIN, "channel_samples": SLICE_WORK}, "epochs_cap": epochs_cap}

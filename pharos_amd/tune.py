@@ -1,0 +1,98 @@
+"""The engine's own autotuner for the batched launch shape.
+
+The tunable knobs are those of pharos_synth_tuning:
+- persistent workgroups per compute unit (1-4: the LDS of a tile allows at most four);
+- channels staged per LDS pass of a tile (4-16).
+
+For a workload shape, the tuner times every candidate on a short sample of the workload itself: the first
+`sample_epochs` epochs, one batched launch, HIP events on the launch stream. It keeps the fastest candidate in a
+JSON cache keyed by the shape. The shape key is (samples per epoch, format, channel count rounded up to a multiple
+of 8, job count bucket). Every candidate produces the same bytes, so tuning changes speed only.
+"""
+import json
+import os
+import time
+
+from . import _lib
+
+WORKGROUPS = (1, 2, 3, 4)
+CHANNELS_PER_PASS = (4, 8, 12, 16)
+CANDIDATES = [(w, c) for w in WORKGROUPS for c in CHANNELS_PER_PASS]
+
+
+def shape_key(scenarios):
+    """Key of a batch of scenarios: the dominant (largest) scenario's rate, format and channel bucket, and the job
+    count's power-of-two bucket."""
+    big = max(scenarios, key=lambda s: s.channel_samples)
+    nch = max(1, len(big.channels))
+    jobs = 1
+    while jobs < len(scenarios):
+        jobs *= 2
+    return f"{big.samples_per_epoch}/{big.format}/{(nch + 7) // 8 * 8}/{jobs}"
+
+
+def _time(jobs, w, c, stream, repeats):
+    from .engine import Event
+    import ctypes as C
+    L = _lib.lib()
+    best = None
+    for _ in range(repeats):
+        for j in jobs:
+            j.agc.gain, j.agc.clipped, j.agc.components, j.agc.adjustments = 1.0, 0, 0, 0
+        arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
+        tuning = _lib.SynthTuning(w, c, (C.c_uint32 * 2)())
+        e0, e1 = Event(), Event()
+        h = C.c_void_p()
+        e0.record(stream)
+        _lib.check(L.pharos_synth_batch_submit(arr, len(jobs), C.byref(tuning), stream.handle, C.byref(h)))
+        e1.record(stream)
+        _lib.check(L.pharos_synth_batch_wait(h))
+        ms = e0.elapsed_ms(e1)
+        best = ms if best is None else min(best, ms)
+    return best
+
+
+def tune(scenarios, sample_epochs=300, repeats=2, cache_path=None, stream=None):
+    """Times every candidate on the first `sample_epochs` epochs of `scenarios` (one batch) and returns
+    {"key", "best": (workgroups_per_cu, channels_per_pass), "times_ms": {...}, "seconds"}; stores the winner in
+    the cache when one is given."""
+    from .engine import Job, Stream
+    t0 = time.perf_counter()
+    stream = stream or Stream()
+    jobs = [Job(s, epochs=min(s.epochs, sample_epochs)) for s in scenarios]
+    _time(jobs, 0, 0, stream, 1)  # warm the code and the caches
+    times = {f"{w}x{c}": _time(jobs, w, c, stream, repeats) for w, c in CANDIDATES}
+    best = min(times, key=times.get)
+    w, c = (int(x) for x in best.split("x"))
+    result = {"key": shape_key(scenarios), "best": (w, c), "times_ms": times, "sample_epochs": sample_epochs,
+              "seconds": time.perf_counter() - t0}
+    if cache_path:
+        store(cache_path, result["key"], (w, c), times[best])
+    for j in jobs:
+        j.engine.close()
+    return result
+
+
+def load(cache_path):
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            return json.load(f)
+    return {}
+
+
+def store(cache_path, key, best, ms):
+    table = load(cache_path)
+    table[key] = {"workgroups_per_cu": best[0], "channels_per_pass": best[1], "sample_ms": ms}
+    os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
+    tmp = cache_path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, cache_path)
+
+
+def lookup(scenarios, cache_path):
+    """(workgroups_per_cu, channels_per_pass) cached for this shape, or (0, 0): the library's defaults."""
+    entry = load(cache_path).get(shape_key(scenarios))
+    if not entry:
+        return 0, 0
+    return int(entry["workgroups_per_cu"]), int(entry["channels_per_pass"])

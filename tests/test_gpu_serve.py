@@ -10,7 +10,10 @@ from tests.test_gpu_batch import _strong_iq4
 pytestmark = pytest.mark.gpu
 
 
-def test_requests_in_the_mix_equal_alone():
+def test_requests_in_the_mix_equal_alone(monkeypatch):
+    # Short slices, so each request crosses several batches and carries its AGC state between them.
+    monkeypatch.setattr(serve, "SLICE_MIN", 300)
+    monkeypatch.setattr(serve, "SLICE_WORK", 1.0)
     reqs = serve.make_requests(rate_hz=400.0, duration_s=0.02, seed=3, epochs_cap=1400)
     reqs.append(serve.Request(_strong_iq4(8, spe=16_368, epochs=2300), 0.004, len(reqs)))
     print("case serve mix", [(r.scenario.samples_per_epoch, r.scenario.format, len(r.scenario.channels),
