@@ -83,10 +83,12 @@ struct BatchState {
     uint32_t nsegs;
     uint64_t total_items;          // end of this launch's items (they start where its queue starts)
     unsigned long long *queue;     // next work item
-    uint32_t *tiles_done;          // per segment
-    unsigned long long *seg_clip;  // per segment
-    float *seg_gain;               // per segment
-    uint32_t *seg_ready;           // per segment
+    // Per segment, two 64-bit words, each read and written only as a whole with relaxed device-scope atomics,
+    // so the per-item path needs no fence:
+    //   seg_count: finished tiles << COUNT_SHIFT | clipped components of those tiles
+    //   seg_gain:  READY_BIT | f32 bits of the gain the segment quantises with
+    unsigned long long *seg_count;
+    unsigned long long *seg_gain;
     DevAgc *agc;                   // per job
     DevAdjust *adjust;             // per job, adjust_cap each
     uint32_t adjust_cap;
@@ -137,18 +139,28 @@ __device__ void agc_step(DevAgc &G, unsigned long long clipped, unsigned long lo
     }
 }
 
-// The AGC step of a finished segment, run by its last tile; publishes the next segment's gain.
+constexpr uint32_t COUNT_SHIFT = 40;                   // a segment clips fewer than 2^40 components
+constexpr unsigned long long CLIP_MASK = (1ull << COUNT_SHIFT) - 1;
+constexpr unsigned long long READY_BIT = 1ull << 32;
+
+__device__ __forceinline__ void publish_gain(const BatchState &S, uint32_t seg, float gain) {
+    __hip_atomic_store(&S.seg_gain[seg], READY_BIT | __float_as_uint(gain), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The AGC step of a finished segment, run by its last tile with the segment's clip total; publishes the next
+// segment's gain.  The job's AGC state is ordinary memory written by one segment's last tile after another: this
+// tile read the gain word the previous writer published after a release fence, so an acquire fence here makes that
+// writer's state visible (once per segment, not per tile).
 __device__ void agc_account(const BatchState &S, uint64_t first_epoch, uint32_t spe, uint32_t job, const DevSegment &seg,
-                            uint32_t s) {
-    unsigned long long clipped = __hip_atomic_load(&S.seg_clip[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                            uint32_t s, unsigned long long clipped) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     DevAgc &G = S.agc[job];
     agc_step(G, clipped, (unsigned long long)seg.epochs * spe * 2ull, first_epoch + seg.epoch_off + seg.epochs,
              S.adjust + (size_t)job * S.adjust_cap, S.adjust_cap);
     const uint32_t nxt = S.seg_next[s];
-    if (nxt != 0xffffffffu) {
-        S.seg_gain[nxt] = (float)G.gain;
-        __hip_atomic_store(&S.seg_ready[nxt], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (nxt != 0xffffffffu) publish_gain(S, nxt, (float)G.gain);
 }
 
 // Graph form: the AGC state enters from `initial` and the first window's gain is published to gain_dev.
@@ -185,7 +197,8 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
     extern __shared__ __align__(16) uint8_t smem[];
     // Queue words live after the largest layout of any job (synth_tile's reduction words sit below them).
     // q[0..1]: the item, q[2]: its segment, q[3]: the gain broadcast.
-    volatile uint32_t *q = reinterpret_cast<volatile uint32_t *>(smem + S.queue_words_offset);
+    typedef volatile __attribute__((address_space(3))) uint32_t lds_word;
+    lds_word *q = (lds_word *)(uintptr_t)lds_address(smem + S.queue_words_offset);
     uint32_t hint = 0;  // thread 0's segment cursor
     if (threadIdx.x == 0) {
         const unsigned long long item = atomicAdd(S.queue, 1ull);
@@ -229,9 +242,10 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
         auto gain_fn = [&]() -> float {
             if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT) return J.constant_gain;
             if (threadIdx.x == 0) {
-                while (__hip_atomic_load(&S.seg_ready[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                unsigned long long w;
+                while (!((w = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & READY_BIT))
                     __builtin_amdgcn_s_sleep(8);
-                q[3] = __float_as_uint(__hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                q[3] = (uint32_t)w;
             }
             __syncthreads();
             return __uint_as_float(q[3]);
@@ -248,17 +262,13 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
         };
         const uint32_t clips = synth_tile<F>(A, smem, gain_fn, after_sums, S.err);
         if (threadIdx.x == 0) {
-            atomicAdd(&S.seg_clip[s], (unsigned long long)clips);
-            __threadfence();
-            const uint32_t done = atomicAdd(&S.tiles_done[s], 1u) + 1u;
-            if (done == seg.epochs * seg.tiles) {
-                __threadfence();
-                if (J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) agc_account(S, J.first_epoch, gmem(J.engine)->spe, seg.job, seg, s);
-                else {
-                    const uint32_t nxt = S.seg_next[s];
-                    if (nxt != 0xffffffffu) __hip_atomic_store(&S.seg_ready[nxt], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
+            // One atomic counts the tile and its clips; the last tile of the segment gets the segment's total.
+            const unsigned long long add = (1ull << COUNT_SHIFT) | clips;
+            const unsigned long long now =
+                __hip_atomic_fetch_add(&S.seg_count[s], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + add;
+            if ((now >> COUNT_SHIFT) == (unsigned long long)seg.epochs * seg.tiles &&
+                J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE)
+                agc_account(S, J.first_epoch, gmem(J.engine)->spe, seg.job, seg, s, now & CLIP_MASK);
         }
     }
 }
@@ -534,7 +544,7 @@ struct pharos_synth_batch_handle {
     std::vector<size_t> owned_bytes;
     DevAgc *d_agc = nullptr;
     DevAdjust *d_adjust = nullptr;
-    unsigned long long *d_seg_clip = nullptr;
+    unsigned long long *d_seg_count = nullptr;
     uint32_t nsegs = 0, adjust_cap = 0;
     std::vector<uint32_t> job_first_seg, job_seg_count, seg_job;
     uint32_t *d_err = nullptr;
@@ -718,16 +728,19 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     auto carve = [&](size_t n) { size_t o = (total + 255) & ~(size_t)255; total = o + n; return Piece{o, n}; };
     Piece p_jobs = carve(sizeof(DevJob) * njobs), p_segs = carve(sizeof(DevSegment) * nsegs),
           p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long) * ngroups),
-          p_done = carve(sizeof(uint32_t) * nsegs), p_clip = carve(sizeof(unsigned long long) * nsegs),
-          p_gain = carve(sizeof(float) * nsegs), p_ready = carve(sizeof(uint32_t) * nsegs),
+          p_count = carve(sizeof(unsigned long long) * nsegs), p_gain = carve(sizeof(unsigned long long) * nsegs),
           p_agc = carve(sizeof(DevAgc) * njobs), p_adj = carve(sizeof(DevAdjust) * (size_t)adjust_cap * njobs),
           p_err = carve(64);
     std::vector<uint8_t> host(total, 0);
     memcpy(host.data() + p_jobs.off, dj.data(), p_jobs.size);
     memcpy(host.data() + p_segs.off, segs.data(), p_segs.size);
     memcpy(host.data() + p_next.off, seg_next.data(), p_next.size);
-    memcpy(host.data() + p_gain.off, seg_gain.data(), p_gain.size);
-    memcpy(host.data() + p_ready.off, seg_ready.data(), p_ready.size);
+    for (uint32_t k = 0; k < nsegs; k++) {
+        uint32_t bits;
+        memcpy(&bits, &seg_gain[k], sizeof bits);
+        const unsigned long long w = (seg_ready[k] ? READY_BIT : 0ull) | bits;
+        memcpy(host.data() + p_gain.off + k * sizeof w, &w, sizeof w);
+    }
     memcpy(host.data() + p_agc.off, agc.data(), p_agc.size);
     for (uint32_t g = 0; g < ngroups; g++) {
         const unsigned long long q0 = groups[g].begin;
@@ -750,10 +763,8 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.nsegs = nsegs;
     S.total_items = items;
     S.queue = nullptr;
-    S.tiles_done = (uint32_t *)(b8 + p_done.off);
-    S.seg_clip = (unsigned long long *)(b8 + p_clip.off);
-    S.seg_gain = (float *)(b8 + p_gain.off);
-    S.seg_ready = (uint32_t *)(b8 + p_ready.off);
+    S.seg_count = (unsigned long long *)(b8 + p_count.off);
+    S.seg_gain = (unsigned long long *)(b8 + p_gain.off);
     S.agc = (DevAgc *)(b8 + p_agc.off);
     S.adjust = (DevAdjust *)(b8 + p_adj.off);
     S.adjust_cap = adjust_cap;
@@ -762,7 +773,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.chunk_cap = tuning ? std::min<uint32_t>(tuning->channels_per_pass, MAX_CHUNK) : 0;
     h->d_agc = S.agc;
     h->d_adjust = S.adjust;
-    h->d_seg_clip = S.seg_clip;
+    h->d_seg_count = S.seg_count;
     h->d_err = S.err;
 
     for (uint32_t g = 0; g < ngroups; g++) {
@@ -822,20 +833,20 @@ extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
     const uint32_t njobs = (uint32_t)h->jobs.size();
     std::vector<DevAgc> agc(njobs);
     std::vector<DevAdjust> adj((size_t)h->adjust_cap * njobs);
-    std::vector<unsigned long long> seg_clip(h->nsegs);
+    std::vector<unsigned long long> seg_count(h->nsegs);
     uint32_t err = 0;
     int status = ST_OK;
     if (hipEventSynchronize(h->done) != hipSuccess ||
         hipMemcpy(agc.data(), h->d_agc, sizeof(DevAgc) * njobs, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(adj.data(), h->d_adjust, sizeof(DevAdjust) * adj.size(), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(seg_clip.data(), h->d_seg_clip, sizeof(unsigned long long) * h->nsegs, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(seg_count.data(), h->d_seg_count, sizeof(unsigned long long) * h->nsegs, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(&err, h->d_err, sizeof err, hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("batch failed: %s", hipGetErrorString(hipGetLastError()));
         batch_free(h);
         return ST_INTERNAL;
     }
     std::vector<uint64_t> job_clip(njobs, 0);
-    for (uint32_t s = 0; s < h->nsegs; s++) job_clip[h->seg_job[s]] += seg_clip[s];
+    for (uint32_t s = 0; s < h->nsegs; s++) job_clip[h->seg_job[s]] += seg_count[s] & CLIP_MASK;
     for (uint32_t j = 0; j < njobs; j++) {
         const pharos_synth_job &J = h->jobs[j];
         if (J.clipped) *J.clipped = job_clip[j];
