@@ -66,6 +66,7 @@ struct ChanRec {
     uint32_t pad[3];
 };
 static_assert(sizeof(ChanRec) % 16 == 0, "ChanRec keeps 16-byte alignment");
+static_assert(THREADS == MAX_CHUNK * TILE_BLOCKS, "a pass sets up each channel with one group of TILE_BLOCKS threads");
 
 // Error bits the kernels raise (read back by the host).
 enum : uint32_t {

@@ -217,28 +217,103 @@ __device__ __forceinline__ void divmod32(uint64_t x, uint32_t d, uint64_t &q, ui
     }
 }
 
-// Setup of one channel for one tile, one thread per channel of the chunk, from its row (already in LDS): the
-// code edge, staged window, shape and symbol masks, and Rotor.init (rotor.zig:21-35) -- the lane powers by the
-// reference's f64 recursion and the f32 16-sample step.  Every global load it needs is independent of the
-// others' results but the table's period, so they are in flight together.
-__device__ __forceinline__ void chan_setup(ChanRec &R, float2 *vt, const pharos_synth_row &row, const DevEngine &E,
-                                           uint32_t n0, uint32_t tile_len, const uint8_t *masks, uint64_t mask_len,
-                                           uint32_t *err) {
-    R.active = 0;
+// Staged dword d of a channel, byte by byte: the entry's value index, as 8 x the index (the value's byte offset in
+// the channel's table) or, for a lead channel, the bare index the lead combines (kernel.zig:227-235).  Plain: the
+// state with its period's symbol mask applied (kernel.zig:183).  TDM (kernel.zig:197-203): odd entries carry
+// 2 | the pilot chip of their period, even ones the state with the mask's low bit.  An index past the last value
+// stages as 8, a zero entry, where the reference would index out of its table.
+__device__ __forceinline__ uint32_t stage_bytes(uint32_t shape, uint32_t m4, uint32_t e0, uint32_t k_lo, uint32_t table,
+                                                uint64_t period0, const DevEngine &E, uint32_t v, uint32_t d) {
+    const uint32_t head = e0 & 3u;
+    uint32_t o = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+        const uint32_t x = (v >> (8 * q)) & 0xffu;
+        const uint32_t w = x >> 6, st = (x >> 3) & 7u;
+        const uint32_t m = (m4 >> (8 * w)) & 0xffu;
+        uint32_t idx = st ^ m;
+        if (shape == SHAPE_TDM) {
+            const auto &T = gmem(E.tables)[table];
+            const int32_t j = (int32_t)(4 * d + q) - (int32_t)head;
+            const uint32_t e = e0 + (uint32_t)(j < 0 ? 0 : j) - w * T.per;
+            if (j >= 0 && (e & 1u)) {
+                const uint64_t period = period0 + k_lo + w;
+                const uint32_t pi = (uint32_t)(period % T.tdm_periods) * T.tdm_half + e / 2;
+                const uint32_t chip = pi < T.pilot_len ? gmem(T.pilot)[pi] : 0u;
+                idx = 2u | chip;
+            } else {
+                idx = st ^ (m & 1u);
+            }
+        }
+        idx = idx < 8u ? idx : 8u;
+        o |= (shape == SHAPE_LEAD ? idx : 8u * idx) << (8 * q);
+    }
+    return o;
+}
+
+// Dwords j, j + group, ... of a channel's staged window, eight loads in flight per thread.  With every mask below 8
+// a dword's four bytes map at once: v_perm picks each byte's period mask by its period offset; TDM windows and
+// out-of-range masks take the byte-wise `slow` path.
+template <class StageFn>
+__device__ __forceinline__ void stage_window(uint32_t *dst, const uint32_t *src_, uint32_t ndw, uint32_t j, uint32_t group,
+                                             uint32_t shape, uint32_t m4, bool wide, StageFn slow) {
+    const gptr<uint32_t> src = gmem(src_);
+    if (wide || shape == SHAPE_TDM) {
+        for (uint32_t d = j; d < ndw; d += group) dst[d] = slow(src[d], d);
+        return;
+    }
+    const bool lead = shape == SHAPE_LEAD;
+    const uint32_t m = lead ? m4 : m4 << 3;
+    const uint32_t keep = lead ? 0x07070707u : 0x38383838u;
+    const uint32_t sh = lead ? 3u : 0u;
+    for (uint32_t d0 = j; d0 < ndw; d0 += 8 * group) {
+        uint32_t v[8];
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) {
+            const uint32_t d = d0 + k * group;
+            v[k] = d < ndw ? src[d] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) {
+            const uint32_t d = d0 + k * group;
+            if (d < ndw) dst[d] = ((v[k] >> sh) & keep) ^ __builtin_amdgcn_perm(m, m, (v[k] >> 6) & 0x03030303u);
+        }
+    }
+}
+
+// Setup and staging of one channel for one tile, by its group of TILE_BLOCKS threads (thread j of the group).  Every
+// thread of the group derives the channel's code edge, staged window, shape and symbol masks from the row (in LDS)
+// in registers, so the group stages the window with no barrier between; then thread j writes
+//   - the seed of rotor block j, sincos64 of the block's start phase (Rotor.seed, rotor.zig:37-38),
+//   - lane powers 2j and 2j+1 of Rotor.init (rotor.zig:21-35; each by the reference's f64 recursion),
+//   - entries 2j and 2j+1 of the channel's value table, f32(f64(amp) * value) (kernel.zig:104-110),
+//   - dwords j, j + 8, ... of the staged window,
+// and thread 0 the record's scalars.
+__device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *stage, const pharos_synth_row &row,
+                                           const DevEngine &E, uint32_t n0, uint32_t tile_len, const uint8_t *masks,
+                                           uint64_t mask_len, uint32_t *err, uint32_t j) {
+    const bool first_lane = j == 0;
+    if (first_lane) R.active = 0;
     if (row.table == PHAROS_SYNTH_INACTIVE) return;
-    if (row.table >= E.ntables) { atomicOr(err, ERR_TABLE); return; }
+    {
+        double c, s;
+        sincos64(row.carrier_phase0 + (double)(n0 + j * RESEED) * row.carrier_step, E.turn, c, s);
+        R.c0_c[j] = c;
+        R.c0_s[j] = s;
+    }
+    if (row.table >= E.ntables) { if (first_lane) atomicOr(err, ERR_TABLE); return; }
     const auto &T = gmem(E.tables)[row.table];
     const uint32_t per = T.per, w_max = T.w_max, ext_len = T.ext_len;
     const uint32_t axis = T.axis, tdm = T.tdm_periods, count = T.count;
     const uint8_t *ext = T.ext;
-    double vre[8], vim[8];
-#pragma unroll
-    for (uint32_t k = 0; k < 8; k++) { vre[k] = T.vre[k]; vim[k] = T.vim[k]; }
+    const uint32_t k0 = 2 * j;  // value-table entries of this thread (the table has at most 8 values)
+    const double vre0 = k0 < 8 ? T.vre[k0] : 0.0, vim0 = k0 < 8 ? T.vim[k0] : 0.0;
+    const double vre1 = k0 + 1 < 8 ? T.vre[k0 + 1] : 0.0, vim1 = k0 + 1 < 8 ? T.vim[k0 + 1] : 0.0;
     double w1c, w1s, wc, ws;
     sincos64(row.carrier_step, E.turn, w1c, w1s);
     sincos64(16.0 * row.carrier_step, E.turn, wc, ws);
     if (!(row.entries_per_sample > 0.0) || !(row.entries_per_sample < 64.0) || !(row.entry_phase0 >= 0.0)) {
-        atomicOr(err, ERR_TABLE);
+        if (first_lane) atomicOr(err, ERR_TABLE);
         return;
     }
     // chip.Edge.of (chip.zig:17-24)
@@ -246,14 +321,17 @@ __device__ __forceinline__ void chan_setup(ChanRec &R, float2 *vt, const pharos_
     const uint64_t first = (uint64_t)fl;
     const uint64_t phase_q = frac_q(row.entry_phase0 - fl);
     const uint64_t eps_q = (uint64_t)round(row.entries_per_sample * 4294967296.0);
-    if (eps_q == 0) { atomicOr(err, ERR_TABLE); return; }
+    if (eps_q == 0) { if (first_lane) atomicOr(err, ERR_TABLE); return; }
     const uint64_t P0 = phase_q + (uint64_t)n0 * eps_q;
     const uint64_t Pl = phase_q + (uint64_t)(n0 + tile_len - 1) * eps_q;
     const uint64_t e_lo = first + (P0 >> 32);
     const uint32_t shape = tdm ? SHAPE_TDM : (row.lead_bits ? SHAPE_LEAD : SHAPE_PLAIN);
     uint32_t lead_whole = 0, lead_q = 0;
     if (shape == SHAPE_LEAD) {
-        if (!(row.lead_entries >= 0.0) || !(row.lead_entries < (double)LEAD_ALLOW)) { atomicOr(err, ERR_STAGE); return; }
+        if (!(row.lead_entries >= 0.0) || !(row.lead_entries < (double)LEAD_ALLOW)) {
+            if (first_lane) atomicOr(err, ERR_STAGE);
+            return;
+        }
         const double lw = floor(row.lead_entries);
         lead_whole = (uint32_t)lw;
         lead_q = frac_q(row.lead_entries - lw);  // chip.zig:162-165
@@ -263,7 +341,7 @@ __device__ __forceinline__ void chan_setup(ChanRec &R, float2 *vt, const pharos_
     uint64_t k_lo;
     uint32_t e0;
     divmod32(e_lo, per, k_lo, e0);
-    if (W > w_max || e0 + W > ext_len) { atomicOr(err, ERR_STAGE); return; }
+    if (W > w_max || e0 + W > ext_len) { if (first_lane) atomicOr(err, ERR_STAGE); return; }
     // The last entry a live sample reads (the lead adds at most one carry), and the period it falls in: the
     // reference's mask slice covers masksNeeded periods (parameters.zig:25-29), so a row reading past the host's
     // mask bytes is a host error.
@@ -273,7 +351,7 @@ __device__ __forceinline__ void chan_setup(ChanRec &R, float2 *vt, const pharos_
     divmod32(e0 + last_rel, per, w_used, unused);
     (void)unused;
     if (k_lo + w_used >= row.mask_count || (uint64_t)row.mask_offset + k_lo + w_used >= mask_len) {
-        atomicOr(err, ERR_MASKS);
+        if (first_lane) atomicOr(err, ERR_MASKS);
         return;
     }
     // The symbol masks of the (at most four) periods the window touches; zero past the last one it reads.
@@ -281,44 +359,55 @@ __device__ __forceinline__ void chan_setup(ChanRec &R, float2 *vt, const pharos_
 #pragma unroll
     for (uint32_t w = 0; w < 4; w++)
         if (w <= w_used) m4 |= (uint32_t)gmem(masks)[(uint64_t)row.mask_offset + k_lo + w] << (8 * w);
-    // The channel's values scaled by its amplitude (kernel.zig:104-110); entries from count on stay zero.
-#pragma unroll
-    for (uint32_t k = 0; k < 8; k++)
-        vt[k] = k < count ? make_float2((float)((double)row.amp * vre[k]), (float)((double)row.amp * vim[k]))
-                          : make_float2(0.0f, 0.0f);
-#pragma unroll
-    for (uint32_t k = 8; k < 16; k++) vt[k] = make_float2(0.0f, 0.0f);
     const uint32_t head = e0 & 3u;
-    R.src = reinterpret_cast<const uint32_t *>(ext + (e0 - head));
-    R.ndw = (uint32_t)((head + W + 3) / 4);
-    R.m4 = m4;
-    R.wide_mask = (m4 & 0xf8f8f8f8u) != 0;
-    R.shape = shape;
-    R.axis = axis;
-    R.table = row.table;
-    R.P0 = P0;
-    R.eps_q = eps_q;
-    R.e0 = e0;
-    R.k_lo = (uint32_t)k_lo;
-    R.W = (uint32_t)W;
-    R.lead_q = lead_q;
-    R.lead_whole = lead_whole;
-    R.lead_bits = row.lead_bits;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(ext + (e0 - head));
+    const uint32_t ndw = (uint32_t)((head + W + 3) / 4);
+    const bool wide = (m4 & 0xf8f8f8f8u) != 0;
+    stage_window(stage, src, ndw, j, TILE_BLOCKS, shape, m4, wide, [&](uint32_t v, uint32_t d) {
+        return stage_bytes(shape, m4, e0, (uint32_t)k_lo, row.table, row.period0, E, v, d);
+    });
+    // Value-table entries 2j and 2j+1; entries from count on stay zero.
+    vt[k0] = k0 < count ? make_float2((float)((double)row.amp * vre0), (float)((double)row.amp * vim0))
+                        : make_float2(0.0f, 0.0f);
+    vt[k0 + 1] = k0 + 1 < count ? make_float2((float)((double)row.amp * vre1), (float)((double)row.amp * vim1))
+                                : make_float2(0.0f, 0.0f);
+    // Lane powers 2j and 2j+1 of the recursion pw[l] = pw[l-1] * w1 (rotor.zig:25-30).
     double ac = 1.0, as = 0.0;
-    R.pw_c[0] = 1.0;
-    R.pw_s[0] = 0.0;
+    if (first_lane) {
+        R.pw_c[0] = 1.0;
+        R.pw_s[0] = 0.0;
+    }
 #pragma unroll
     for (uint32_t l = 1; l < LANES16; l++) {
         const double nc = ac * w1c - as * w1s;
         const double ns = ac * w1s + as * w1c;
         ac = nc;
         as = ns;
-        R.pw_c[l] = ac;
-        R.pw_s[l] = as;
+        if (l == k0 || l == k0 + 1) {
+            R.pw_c[l] = ac;
+            R.pw_s[l] = as;
+        }
     }
-    R.w_re = (float)wc;
-    R.w_im = (float)ws;
-    R.active = 1;
+    if (first_lane) {
+        R.src = src;
+        R.ndw = ndw;
+        R.m4 = m4;
+        R.wide_mask = wide;
+        R.shape = shape;
+        R.axis = axis;
+        R.table = row.table;
+        R.P0 = P0;
+        R.eps_q = eps_q;
+        R.e0 = e0;
+        R.k_lo = (uint32_t)k_lo;
+        R.W = (uint32_t)W;
+        R.lead_q = lead_q;
+        R.lead_whole = lead_whole;
+        R.lead_bits = row.lead_bits;
+        R.w_re = (float)wc;
+        R.w_im = (float)ws;
+        R.active = 1;
+    }
 }
 
 // The per-sample loop of one channel for one thread: 32 samples at stride 16 of one rotor block.
@@ -477,40 +566,6 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
     return clip;
 }
 
-// Staged dword d of a channel, byte by byte: the entry's value index, as 8 x the index (the value's byte offset in
-// the channel's table) or, for a lead channel, the bare index the lead combines (kernel.zig:227-235).  Plain: the
-// state with its period's symbol mask applied (kernel.zig:183).  TDM (kernel.zig:197-203): odd entries carry
-// 2 | the pilot chip of their period, even ones the state with the mask's low bit.  An index past the last value
-// stages as 8, a zero entry, where the reference would index out of its table.
-__device__ __forceinline__ uint32_t stage_bytes(const ChanRec &R, const pharos_synth_row &row, const DevEngine &E,
-                                                uint32_t v, uint32_t d) {
-    const uint32_t head = R.e0 & 3u;
-    uint32_t o = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < 4; q++) {
-        const uint32_t x = (v >> (8 * q)) & 0xffu;
-        const uint32_t w = x >> 6, st = (x >> 3) & 7u;
-        const uint32_t m = (R.m4 >> (8 * w)) & 0xffu;
-        uint32_t idx = st ^ m;
-        if (R.shape == SHAPE_TDM) {
-            const auto &T = gmem(E.tables)[R.table];
-            const int32_t j = (int32_t)(4 * d + q) - (int32_t)head;
-            const uint32_t e = R.e0 + (uint32_t)(j < 0 ? 0 : j) - w * T.per;
-            if (j >= 0 && (e & 1u)) {
-                const uint64_t period = row.period0 + R.k_lo + w;
-                const uint32_t pi = (uint32_t)(period % T.tdm_periods) * T.tdm_half + e / 2;
-                const uint32_t chip = pi < T.pilot_len ? gmem(T.pilot)[pi] : 0u;
-                idx = 2u | chip;
-            } else {
-                idx = st ^ (m & 1u);
-            }
-        }
-        idx = idx < 8u ? idx : 8u;
-        o |= (R.shape == SHAPE_LEAD ? idx : 8u * idx) << (8 * q);
-    }
-    return o;
-}
-
 // The engine descriptor by value, read through a global view (the pointer may come from memory).
 __device__ __forceinline__ DevEngine load_engine(const DevEngine *p) {
     const auto &g = *gmem(p);
@@ -576,56 +631,13 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
     __syncthreads();
     for (uint32_t c0 = 0; c0 < A.channels; c0 += E.chunk) {
         const uint32_t nc = min(E.chunk, A.channels - c0);
-        // Stage A.  Block seeds, sincos64 of each block's start (Rotor.seed, rotor.zig:37-38), on every thread;
-        // the per-channel setup on one thread per channel.
-        for (uint32_t i = tid; i < nc * TILE_BLOCKS; i += THREADS) {
-            const pharos_synth_row &row = L.rows[i / TILE_BLOCKS];
-            if (row.table == PHAROS_SYNTH_INACTIVE) continue;
-            const uint32_t bb = i % TILE_BLOCKS;
-            double c, s;
-            sincos64(row.carrier_phase0 + (double)(n0 + bb * RESEED) * row.carrier_step, E.turn, c, s);
-            L.rec[i / TILE_BLOCKS].c0_c[bb] = c;
-            L.rec[i / TILE_BLOCKS].c0_s[bb] = s;
-        }
-        if (tid < nc)
-            chan_setup(L.rec[tid], reinterpret_cast<float2 *>(L.vt + tid * VT_BYTES), L.rows[tid], E, n0, tile_len,
-                       A.masks, A.mask_len, err);
-        __syncthreads();
-        // Stage B.  Every channel's entries of this tile, staged from its extended table as value offsets, by a
-        // group of STAGE_GROUP threads per channel that reads the channel's descriptor once and keeps eight loads
-        // in flight.  With every mask below 8 a dword's four bytes map at once: v_perm picks each byte's period
-        // mask by its period offset.
+        // Setup and staging: one group of TILE_BLOCKS threads per channel of the chunk.
         {
-            constexpr uint32_t STAGE_GROUP = THREADS / MAX_CHUNK;
-            const uint32_t c = tid / STAGE_GROUP, j = tid % STAGE_GROUP;
-            const ChanRec &R = L.rec[c < nc ? c : 0];
-            const uint32_t ndw = c < nc && R.active ? R.ndw : 0u;
-            if (ndw) {
-                const gptr<uint32_t> src = gmem(R.src);
-                uint32_t *dst = reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot);
-                if (R.wide_mask || R.shape == SHAPE_TDM) {
-                    for (uint32_t d = j; d < ndw; d += STAGE_GROUP) dst[d] = stage_bytes(R, L.rows[c], E, src[d], d);
-                } else {
-                    const bool lead = R.shape == SHAPE_LEAD;
-                    const uint32_t m = lead ? R.m4 : R.m4 << 3;
-                    const uint32_t keep = lead ? 0x07070707u : 0x38383838u;
-                    const uint32_t sh = lead ? 3u : 0u;
-                    for (uint32_t d0 = j; d0 < ndw; d0 += 8 * STAGE_GROUP) {
-                        uint32_t v[8];
-#pragma unroll
-                        for (uint32_t k = 0; k < 8; k++) {
-                            const uint32_t d = d0 + k * STAGE_GROUP;
-                            v[k] = d < ndw ? src[d] : 0u;
-                        }
-#pragma unroll
-                        for (uint32_t k = 0; k < 8; k++) {
-                            const uint32_t d = d0 + k * STAGE_GROUP;
-                            if (d < ndw)
-                                dst[d] = ((v[k] >> sh) & keep) ^ __builtin_amdgcn_perm(m, m, (v[k] >> 6) & 0x03030303u);
-                        }
-                    }
-                }
-            }
+            const uint32_t c = tid / TILE_BLOCKS, j = tid % TILE_BLOCKS;
+            if (c < nc)
+                chan_group(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
+                           reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot), L.rows[c], E, n0, tile_len, A.masks,
+                           A.mask_len, err, j);
         }
         __syncthreads();
         // The next chunk's rows travel from global memory while this chunk computes.
