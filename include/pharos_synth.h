@@ -138,10 +138,12 @@ typedef struct pharos_synth_job {
     uint64_t *clipped;                /* host, optional: total clipped components of the job */
 } pharos_synth_job;
 
-/* Launch shape; zero fields take the tuned defaults. */
+/* Launch shape of the batched form; zero fields take the defaults (as many workgroups per compute unit as fit,
+ * and each engine's largest channel pass).  pharos_amd/tune.py chooses them per workload shape. */
 typedef struct pharos_synth_tuning {
     uint32_t workgroups_per_cu;      /* persistent workgroups per compute unit */
-    uint32_t reserved[3];
+    uint32_t channels_per_pass;      /* channels staged in LDS per pass of a tile, 1..32 */
+    uint32_t reserved[2];
 } pharos_synth_tuning;
 
 /* Runs all jobs to completion on `stream` and synchronises it.  Each job's bytes equal what the plain form

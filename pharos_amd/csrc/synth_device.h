@@ -14,7 +14,9 @@ constexpr uint32_t STEPS = RESEED / LANES16;  // 32 advances per block
 constexpr uint32_t TILE_BLOCKS = 8;
 constexpr uint32_t THREADS = TILE_BLOCKS * LANES16;  // 128 = two wave64
 constexpr uint32_t TILE = TILE_BLOCKS * RESEED;      // 4096 samples
-constexpr uint32_t MAX_CHUNK = 16;   // channels staged per LDS pass (at most THREADS / 8: one uint4 of a row per thread)
+constexpr uint32_t MAX_CHUNK = 32;   // channels staged per LDS pass (at most THREADS / 4: one uint4 of a row per thread)
+constexpr uint32_t GROUP = THREADS / MAX_CHUNK;  // threads that set up and stage one channel of a pass
+constexpr uint32_t LDS_LIMIT = 40960;  // LDS per workgroup that keeps four workgroups on a CU's 160 KiB
 constexpr uint32_t LEAD_ALLOW = 64;  // entries a leading component may be ahead
 constexpr double RATE_MARGIN = 1.031; // derive() admits 3 % (parameters.zig:46,55)
 constexpr uint32_t VT_BYTES = 128;   // value table: 16 x (re, im) f32, entries from the table's count on zero
@@ -66,7 +68,8 @@ struct ChanRec {
     uint32_t pad[3];
 };
 static_assert(sizeof(ChanRec) % 16 == 0, "ChanRec keeps 16-byte alignment");
-static_assert(THREADS == MAX_CHUNK * TILE_BLOCKS, "a pass sets up each channel with one group of TILE_BLOCKS threads");
+static_assert(THREADS == MAX_CHUNK * GROUP && TILE_BLOCKS % GROUP == 0 && LANES16 % GROUP == 0,
+              "a pass sets up each channel with one group of GROUP threads");
 
 // Error bits the kernels raise (read back by the host).
 enum : uint32_t {

@@ -434,8 +434,9 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
     e->view.tiles = (spe + TILE - 1) / TILE;
     // Slots 32 bytes past a multiple of 64: the eight channels a wave stages together start on distinct banks.
     e->view.w_slot = (std::max<uint32_t>(w_slot_max, 16) + 31) / 64 * 64 + 32;
-    // Channels per LDS pass: as many as fit a 32 KiB budget; the tile's sums reuse the same bytes afterwards.
-    const uint32_t budget = SUM_BYTES - ROWS_BYTES;
+    // Channels per LDS pass: as many as fit next to the rows within LDS_LIMIT, less the reduction and work-queue
+    // words (four workgroups per CU); the tile's sums reuse the same bytes afterwards.
+    const uint32_t budget = LDS_LIMIT - 64 - 16 - ROWS_BYTES;
     const uint32_t per_ch = (uint32_t)sizeof(ChanRec) + VT_BYTES + e->view.w_slot;
     uint32_t chunk = budget / per_ch;
     chunk = std::max<uint32_t>(1, std::min<uint32_t>(chunk, MAX_CHUNK));

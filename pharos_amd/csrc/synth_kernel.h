@@ -174,7 +174,7 @@ __device__ __forceinline__ float2 lds_load_f2(uint32_t a) {
 // The channel region (records, value tables, staged entries) and, after the channel loop, the tile's sums
 // (TILE f32 I then TILE f32 Q) share the same bytes.
 struct Lds {
-    pharos_synth_row *rows;  // the chunk's rows (MAX_CHUNK)
+    pharos_synth_row *rows;  // the chunk's rows (at most MAX_CHUNK)
     ChanRec *rec;
     uint8_t *vt;      // chunk x VT_BYTES: 16 float2 values per channel
     uint8_t *stage;   // chunk x w_slot
@@ -281,34 +281,42 @@ __device__ __forceinline__ void stage_window(uint32_t *dst, const uint32_t *src_
     }
 }
 
-// Setup and staging of one channel for one tile, by its group of TILE_BLOCKS threads (thread j of the group).  Every
-// thread of the group derives the channel's code edge, staged window, shape and symbol masks from the row (in LDS)
-// in registers, so the group stages the window with no barrier between; then thread j writes
-//   - the seed of rotor block j, sincos64 of the block's start phase (Rotor.seed, rotor.zig:37-38),
-//   - lane powers 2j and 2j+1 of Rotor.init (rotor.zig:21-35; each by the reference's f64 recursion),
-//   - entries 2j and 2j+1 of the channel's value table, f32(f64(amp) * value) (kernel.zig:104-110),
-//   - dwords j, j + 8, ... of the staged window,
-// and thread 0 the record's scalars.
+// Setup and staging of one channel for one tile, by its group of G threads (thread j of the group).  Every thread
+// of the group derives the channel's code edge, staged window, shape and symbol masks from the row (in LDS) in
+// registers, so the group stages the window with no barrier between; then thread j writes
+//   - the seeds of rotor blocks j, j + G, ..., sincos64 of each block's start phase (Rotor.seed, rotor.zig:37-38),
+//   - lane powers and value-table entries j * PER .. j * PER + PER - 1, PER = 16 / G: the powers of Rotor.init
+//     (rotor.zig:21-35) each by the reference's f64 recursion, the values f32(f64(amp) * value) (kernel.zig:104-110),
+//   - dwords j, j + G, ... of the staged window,
+// and thread 0 the record's scalars.  A pass of up to THREADS / 8 channels gives each channel 8 threads, a larger
+// one 4.
+template <uint32_t G>
 __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *stage, const pharos_synth_row &row,
                                            const DevEngine &E, uint32_t n0, uint32_t tile_len, const uint8_t *masks,
                                            uint64_t mask_len, uint32_t *err, uint32_t j) {
     const bool first_lane = j == 0;
     if (first_lane) R.active = 0;
     if (row.table == PHAROS_SYNTH_INACTIVE) return;
-    {
+#pragma unroll
+    for (uint32_t b = j; b < TILE_BLOCKS; b += G) {
         double c, s;
-        sincos64(row.carrier_phase0 + (double)(n0 + j * RESEED) * row.carrier_step, E.turn, c, s);
-        R.c0_c[j] = c;
-        R.c0_s[j] = s;
+        sincos64(row.carrier_phase0 + (double)(n0 + b * RESEED) * row.carrier_step, E.turn, c, s);
+        R.c0_c[b] = c;
+        R.c0_s[b] = s;
     }
     if (row.table >= E.ntables) { if (first_lane) atomicOr(err, ERR_TABLE); return; }
     const auto &T = gmem(E.tables)[row.table];
     const uint32_t per = T.per, w_max = T.w_max, ext_len = T.ext_len;
     const uint32_t axis = T.axis, tdm = T.tdm_periods, count = T.count;
     const uint8_t *ext = T.ext;
-    const uint32_t k0 = 2 * j;  // value-table entries of this thread (the table has at most 8 values)
-    const double vre0 = k0 < 8 ? T.vre[k0] : 0.0, vim0 = k0 < 8 ? T.vim[k0] : 0.0;
-    const double vre1 = k0 + 1 < 8 ? T.vre[k0 + 1] : 0.0, vim1 = k0 + 1 < 8 ? T.vim[k0 + 1] : 0.0;
+    constexpr uint32_t PER = LANES16 / G;      // lane powers and value-table entries of each thread
+    const uint32_t k0 = PER * j;               // the table has at most 8 values: entries from 8 on are zero
+    double vre[PER], vim[PER];
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) {
+        vre[i] = k0 + i < 8 ? T.vre[k0 + i] : 0.0;
+        vim[i] = k0 + i < 8 ? T.vim[k0 + i] : 0.0;
+    }
     double w1c, w1s, wc, ws;
     sincos64(row.carrier_step, E.turn, w1c, w1s);
     sincos64(16.0 * row.carrier_step, E.turn, wc, ws);
@@ -363,15 +371,15 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
     const uint32_t *src = reinterpret_cast<const uint32_t *>(ext + (e0 - head));
     const uint32_t ndw = (uint32_t)((head + W + 3) / 4);
     const bool wide = (m4 & 0xf8f8f8f8u) != 0;
-    stage_window(stage, src, ndw, j, TILE_BLOCKS, shape, m4, wide, [&](uint32_t v, uint32_t d) {
+    stage_window(stage, src, ndw, j, G, shape, m4, wide, [&](uint32_t v, uint32_t d) {
         return stage_bytes(shape, m4, e0, (uint32_t)k_lo, row.table, row.period0, E, v, d);
     });
-    // Value-table entries 2j and 2j+1; entries from count on stay zero.
-    vt[k0] = k0 < count ? make_float2((float)((double)row.amp * vre0), (float)((double)row.amp * vim0))
-                        : make_float2(0.0f, 0.0f);
-    vt[k0 + 1] = k0 + 1 < count ? make_float2((float)((double)row.amp * vre1), (float)((double)row.amp * vim1))
-                                : make_float2(0.0f, 0.0f);
-    // Lane powers 2j and 2j+1 of the recursion pw[l] = pw[l-1] * w1 (rotor.zig:25-30).
+    // This thread's value-table entries; entries from count on stay zero.
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++)
+        vt[k0 + i] = k0 + i < count ? make_float2((float)((double)row.amp * vre[i]), (float)((double)row.amp * vim[i]))
+                                    : make_float2(0.0f, 0.0f);
+    // This thread's lane powers of the recursion pw[l] = pw[l-1] * w1 (rotor.zig:25-30).
     double ac = 1.0, as = 0.0;
     if (first_lane) {
         R.pw_c[0] = 1.0;
@@ -383,7 +391,7 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
         const double ns = ac * w1s + as * w1c;
         ac = nc;
         as = ns;
-        if (l == k0 || l == k0 + 1) {
+        if (l >= k0 && l < k0 + PER) {
             R.pw_c[l] = ac;
             R.pw_s[l] = as;
         }
@@ -631,13 +639,19 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
     __syncthreads();
     for (uint32_t c0 = 0; c0 < A.channels; c0 += E.chunk) {
         const uint32_t nc = min(E.chunk, A.channels - c0);
-        // Setup and staging: one group of TILE_BLOCKS threads per channel of the chunk.
-        {
-            const uint32_t c = tid / TILE_BLOCKS, j = tid % TILE_BLOCKS;
+        // Setup and staging: one group of threads per channel of the chunk.
+        if (nc <= THREADS / 8) {
+            const uint32_t c = tid / 8, j = tid % 8;
             if (c < nc)
-                chan_group(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
-                           reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot), L.rows[c], E, n0, tile_len, A.masks,
-                           A.mask_len, err, j);
+                chan_group<8>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
+                              reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot), L.rows[c], E, n0, tile_len, A.masks,
+                              A.mask_len, err, j);
+        } else {
+            const uint32_t c = tid / GROUP, j = tid % GROUP;
+            if (c < nc)
+                chan_group<GROUP>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
+                                  reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot), L.rows[c], E, n0, tile_len,
+                                  A.masks, A.mask_len, err, j);
         }
         __syncthreads();
         // The next chunk's rows travel from global memory while this chunk computes.

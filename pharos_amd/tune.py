@@ -2,7 +2,7 @@
 
 The tunable knobs are those of pharos_synth_tuning:
 - persistent workgroups per compute unit (1-4: the LDS of a tile allows at most four);
-- channels staged per LDS pass of a tile (4-16).
+- channels staged per LDS pass of a tile (8-32).
 
 For a workload shape, the tuner times every candidate on a short sample of the workload itself: the first
 `sample_epochs` epochs, one batched launch, HIP events on the launch stream. It keeps the fastest candidate in a
@@ -16,7 +16,7 @@ import time
 from . import _lib
 
 WORKGROUPS = (1, 2, 3, 4)
-CHANNELS_PER_PASS = (4, 8, 12, 16)
+CHANNELS_PER_PASS = (8, 16, 24, 32)
 CANDIDATES = [(w, c) for w in WORKGROUPS for c in CHANNELS_PER_PASS]
 
 

@@ -17,7 +17,7 @@ def test_launch_shapes_give_identical_results():
     scs = S.c3(seed=7, count=4, epochs_cap=250) + [S.c1(seed=2, epochs=150)]
     print("case tuning shapes", [(s.samples_per_epoch, s.format, len(s.channels)) for s in scs], flush=True)
     ref = _run(scs, 0, 0)
-    for w, c in [(1, 4), (2, 16), (4, 8), (3, 12)]:
+    for w, c in [(1, 4), (2, 16), (4, 8), (3, 12), (4, 32), (2, 24)]:
         got = _run(scs, w, c)
         for (rb, rc, ra, rg), (gb, gc, ga, gg) in zip(ref, got):
             assert np.array_equal(rb, gb), (w, c)
