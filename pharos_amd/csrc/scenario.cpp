@@ -89,15 +89,14 @@ uint64_t masks_needed(const pharos_synth_row &p, uint32_t spe, uint32_t per) {
     return (uint64_t)floor(end / (double)per) + 2;
 }
 
-}  // namespace
+// One endpoint of the channel's range model at receiver time t_rx_ms: carrier phase in cycles and transmit time.
+struct Endpoint {
+    double phase;
+    int64_t tx_ms;
+    double tx_sub;
+};
 
-extern "C" {
-
-void pharos_scenario_endpoint(const pharos_scenario_spec *s, const pharos_channel_spec *c, int64_t t_rx_ms, double *phase_cycles,
-                              int64_t *tx_ms, double *tx_sub_ms) {
-    pharos_signal_constants k;
-    pharos_signal_info(c->signal, &k);
-    const double lambda = LIGHT / carrier_hz(k, *c);
+Endpoint endpoint_at(const pharos_scenario_spec *s, const pharos_channel_spec *c, double lambda, int64_t t_rx_ms) {
     const double t = (double)(t_rx_ms - s->start_gps_ms) / 1000.0;
     double wobble = 0.0;
     if (c->wobble_accel_mps2 != 0.0 && c->wobble_period_s > 0.0) {
@@ -106,24 +105,23 @@ void pharos_scenario_endpoint(const pharos_scenario_spec *s, const pharos_channe
     }
     const double cycles = -(c->doppler_hz * t + 0.5 * c->doppler_rate_hz_s * t * t) + wobble / lambda;
     const double range = c->range0_m + cycles * lambda;
-    *phase_cycles = cycles;
     const double d = range / LIGHT * 1000.0;  // travel time in ms
     const double up = ceil(d);
-    *tx_ms = t_rx_ms - (int64_t)up;
-    *tx_sub_ms = up - d;
+    return {cycles, t_rx_ms - (int64_t)up, up - d};
 }
 
-int pharos_derive(const pharos_scenario_spec *s, const pharos_channel_spec *c, uint32_t table, uint64_t epoch, pharos_synth_row *out) {
-    pharos_signal_constants k;
-    pharos_signal_info(c->signal, &k);
+double amplitude(const pharos_channel_spec *c, uint32_t spe) {
+    return (double)(float)sqrt(2.0 * pow(10.0, c->cn0_dbhz / 10.0) / (1000.0 * (double)spe));  // parameters.zig:78-81
+}
+
+// parameters.derive (parameters.zig:51-70) of one epoch from its two endpoints.
+int derive_from(const pharos_scenario_spec *s, const pharos_channel_spec *c, const pharos_signal_constants &k,
+                uint32_t table, uint64_t epoch, const Endpoint &a, const Endpoint &b, float amp, pharos_synth_row *out) {
     memset(out, 0, sizeof *out);
     out->table = PHAROS_SYNTH_INACTIVE;
-    const int64_t ta = s->start_gps_ms + (int64_t)epoch, tb = ta + 1;
-    double pa, pb, sa, sb;
-    int64_t ma, mb;
-    pharos_scenario_endpoint(s, c, ta, &pa, &ma, &sa);
-    pharos_scenario_endpoint(s, c, tb, &pb, &mb, &sb);
-    // parameters.derive (parameters.zig:51-70)
+    const int64_t ta = s->start_gps_ms + (int64_t)epoch;
+    const double pa = a.phase, pb = b.phase, sa = a.tx_sub, sb = b.tx_sub;
+    const int64_t ma = a.tx_ms, mb = b.tx_ms;
     const double n = (double)s->samples_per_epoch;
     const int64_t bms = base_ms(k.system, ma, s->leap_seconds);
     const double span_ms = (double)(mb - ma) + (sb - sa);
@@ -138,12 +136,35 @@ int pharos_derive(const pharos_scenario_spec *s, const pharos_channel_spec *c, u
     out->entries_per_sample = span_ms * epm / n;
     out->period0 = (uint64_t)period0;
     out->table = table;
-    out->amp = (float)sqrt(2.0 * pow(10.0, c->cn0_dbhz / 10.0) / (1000.0 * n));  // parameters.zig:78-81
+    out->amp = amp;
     if (c->lead_component && c->lead_entries > 0.0) {  // channels.zig:353-356
         out->lead_bits = (uint8_t)(c->lead_component == 1 ? k.state_bits_pilot : k.state_bits_data);
         out->lead_entries = out->lead_bits ? c->lead_entries : 0.0;
     }
     return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pharos_scenario_endpoint(const pharos_scenario_spec *s, const pharos_channel_spec *c, int64_t t_rx_ms, double *phase_cycles,
+                              int64_t *tx_ms, double *tx_sub_ms) {
+    pharos_signal_constants k;
+    pharos_signal_info(c->signal, &k);
+    const Endpoint e = endpoint_at(s, c, LIGHT / carrier_hz(k, *c), t_rx_ms);
+    *phase_cycles = e.phase;
+    *tx_ms = e.tx_ms;
+    *tx_sub_ms = e.tx_sub;
+}
+
+int pharos_derive(const pharos_scenario_spec *s, const pharos_channel_spec *c, uint32_t table, uint64_t epoch, pharos_synth_row *out) {
+    pharos_signal_constants k;
+    pharos_signal_info(c->signal, &k);
+    const double lambda = LIGHT / carrier_hz(k, *c);
+    const int64_t ta = s->start_gps_ms + (int64_t)epoch;
+    return derive_from(s, c, k, table, epoch, endpoint_at(s, c, lambda, ta), endpoint_at(s, c, lambda, ta + 1),
+                       (float)amplitude(c, s->samples_per_epoch), out);
 }
 
 uint64_t pharos_scenario_mask_bound(const pharos_scenario_spec *s, const pharos_channel_spec *channels, uint32_t epochs) {
@@ -168,9 +189,17 @@ int64_t pharos_scenario_window(const pharos_scenario_spec *s, const pharos_chann
         pharos_signal_constants k;
         if (pharos_signal_info(ch.signal, &k)) return -2;
         uint64_t first = UINT64_MAX, last = 0;
+        // Each epoch's end is the next one's start: one endpoint per epoch boundary.
+        const double lambda = LIGHT / carrier_hz(k, ch);
+        const float amp = (float)amplitude(&ch, s->samples_per_epoch);
+        const int64_t t0 = s->start_gps_ms + (int64_t)first_epoch;
+        Endpoint a = endpoint_at(s, &ch, lambda, t0);
         for (uint32_t e = 0; e < epochs; e++) {
             pharos_synth_row &r = rows[(size_t)e * nch + c];
-            if (!pharos_derive(s, &ch, c, first_epoch + e, &r)) continue;
+            const Endpoint b = endpoint_at(s, &ch, lambda, t0 + (int64_t)e + 1);
+            const int ok = derive_from(s, &ch, k, c, first_epoch + e, a, b, amp, &r);
+            a = b;
+            if (!ok) continue;
             first = std::min<uint64_t>(first, r.period0);
             last = std::max<uint64_t>(last, r.period0 + masks_needed(r, s->samples_per_epoch, k.entries_per_period));
         }
