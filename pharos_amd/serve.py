@@ -4,7 +4,7 @@ Requests are runs (scenarios) arriving over time. The loop works in iterations:
 1. It admits every arrived request whose device memory fits the free HBM (pharos_synth_mem_info), oldest first.
 2. It submits one persistent batch holding the next slice of every admitted request. A slice is a whole number
    of 100-epoch AGC windows, sized so it carries about SLICE_WORK channel-samples.
-3. While the GPU runs that batch, the host derives each request's following slice and copies it into the
+3. While the GPU runs that batch, worker threads derive each request's following slice and copy it into the
    request's second row/mask buffer.
 4. It retires the requests that completed.
 
@@ -70,7 +70,8 @@ class Request:
 
     def device_need(self):
         rows, masks = self.slice_bytes()
-        return self.out_bytes + 2 * (rows + masks) + (8 << 20)
+        c = BufferPool._cls
+        return c(self.out_bytes) + 2 * (c(rows) + c(masks)) + (8 << 20)
 
     def gain_mode(self):
         return 1 if self.scenario.gain_db is not None else 0
@@ -92,10 +93,61 @@ def _prepare(req, first):
     return first, n, rows, masks
 
 
-def _arrive(req):
-    """Work done as soon as a request arrives, before it is admitted: its code tables and its first slice."""
-    req.scenario.tables()
+def _arrive(req, device):
+    """Work done as soon as a request arrives, before it is admitted: its engine (code tables on the device, a few
+    MB) and its first slice."""
+    req.engine = Engine(req.scenario, device)
     return _prepare(req, 0)
+
+
+def _copy_into(buffers, prepared):
+    """Copies a prepared slice into a request's row/mask buffers (worker thread)."""
+    first, n, rows, masks = prepared
+    d_rows, d_masks = buffers
+    if rows.nbytes > d_rows.nbytes or masks.nbytes > d_masks.nbytes:
+        raise RuntimeError("slice larger than its buffers")
+    if rows.nbytes:
+        check(_lib.lib().pharos_synth_memcpy(d_rows.ptr, rows.ctypes.data, rows.nbytes, 0))
+    check(_lib.lib().pharos_synth_memcpy(d_masks.ptr, masks.ctypes.data, masks.nbytes, 0))
+    return first, n, d_rows, d_masks, len(masks)
+
+
+def _next_slice(req, first, buffers):
+    """The slice at `first`, derived and copied into `buffers` (one worker task)."""
+    return _copy_into(buffers, _prepare(req, first))
+
+
+class BufferPool:
+    """Device buffers kept for reuse by size class (powers of two), so admissions and retirements do not allocate
+    and free device memory each time."""
+
+    def __init__(self, device, keep_bytes=32 << 30):
+        self.device = device
+        self.keep = keep_bytes
+        self.free = collections.defaultdict(list)
+        self.held = 0
+
+    @staticmethod
+    def _cls(n):
+        return 1 << max(16, (max(int(n), 1) - 1).bit_length())
+
+    def take(self, n):
+        c = self._cls(n)
+        if self.free[c]:
+            self.held -= c
+            return self.free[c].pop()
+        return DeviceBuffer(c, self.device)
+
+    def give(self, buf):
+        c = buf.nbytes
+        if self.held + c <= self.keep:
+            self.free[c].append(buf)
+            self.held += c
+        else:
+            buf.free()
+
+    def pooled(self):
+        return self.held
 
 
 class Server:
@@ -106,38 +158,41 @@ class Server:
         self.tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, (C.c_uint32 * 2)())
         self.pool = ThreadPoolExecutor(max_workers=threads or min(16, os.cpu_count() or 4))
         self.stream = Stream(device)
+        self.buffers = BufferPool(device)
         self.iterations = 0
         self.batch_sizes = []
         self.phase_s = collections.Counter()  # host seconds per loop phase
 
     def _free_hbm(self):
+        """Free device memory, counting what the buffer pool holds as free."""
         free, total = C.c_uint64(), C.c_uint64()
         check(_lib.lib().pharos_synth_mem_info(self.device, C.byref(free), C.byref(total)))
-        return int(free.value)
+        return int(free.value) + self.buffers.pooled()
 
     def _admit(self, req):
-        req.engine = Engine(req.scenario, self.device)
-        req.out = DeviceBuffer(max(req.out_bytes, 16), self.device)
+        req.out = self.buffers.take(req.out_bytes)
         rows, masks = req.slice_bytes()
         # Two row/mask buffers per request: the next slice is written into one while the batch reads the other.
-        req.buffers = [(DeviceBuffer(rows, self.device), DeviceBuffer(masks, self.device)) for _ in range(2)]
+        req.buffers = [(self.buffers.take(rows), self.buffers.take(masks)) for _ in range(2)]
         req.flip = 0
         req.admitted = time.perf_counter()
 
-    def _upload(self, req, prepared):
-        first, n, rows, masks = prepared
-        d_rows, d_masks = req.buffers[req.flip]
+    def _spare(self, req):
+        """The row/mask buffers the next slice goes into (not the ones the running batch reads)."""
+        b = req.buffers[req.flip]
         req.flip ^= 1
-        if rows.nbytes > d_rows.nbytes or masks.nbytes > d_masks.nbytes:
-            raise RuntimeError("slice larger than its buffers")
-        if rows.nbytes:
-            check(_lib.lib().pharos_synth_memcpy(d_rows.ptr, rows.ctypes.data, rows.nbytes, 0))
-        check(_lib.lib().pharos_synth_memcpy(d_masks.ptr, masks.ctypes.data, masks.nbytes, 0))
-        req.staged = (first, n, d_rows, d_masks, len(masks))
+        return b
 
-    def _retire(self, req):
+    def _retire(self, req, keep_output):
+        for pair in req.buffers:
+            for b in pair:
+                self.buffers.give(b)
         req.buffers = None
         req.staged = None
+        if not keep_output:
+            self.buffers.give(req.out)
+            req.out = None
+            req.engine.close()
 
     def _job(self, req):
         first, n, d_rows, d_masks, mask_len = req.staged
@@ -163,7 +218,7 @@ class Server:
             now = time.perf_counter()
             while pending and pending[0].arrival_abs <= now:
                 req = pending.popleft()
-                req.arrived = self.pool.submit(_arrive, req)
+                req.arrived = self.pool.submit(_arrive, req, self.device)
                 waiting.append(req)
             # Admission by free device memory, oldest first, of requests whose first slice is ready.
             admitted = []
@@ -177,8 +232,9 @@ class Server:
                 admitted.append(req)
             tb = time.perf_counter()
             for req in admitted:
-                self._upload(req, req.arrived.result())
+                prepared = req.arrived.result()
                 req.arrived = None
+                req.staged = _copy_into(self._spare(req), prepared)
             active.extend(admitted)
             tc = time.perf_counter()
             self.phase_s["admit"] += tb - ta
@@ -198,9 +254,9 @@ class Server:
             # While it runs: the next slice of every request that has one, derived on the host and copied into the
             # request's other row/mask buffer.
             nxt = [(r, sum(r.current)) for r in active if sum(r.current) < r.epochs]
-            futures = [(r, self.pool.submit(_prepare, r, f)) for r, f in nxt]
+            futures = [(r, self.pool.submit(_next_slice, r, f, self._spare(r))) for r, f in nxt]
             for r, f in futures:
-                self._upload(r, f.result())
+                r.staged = f.result()
             te = time.perf_counter()
             check(L.pharos_synth_batch_wait(h))
             end = time.perf_counter()
@@ -212,13 +268,10 @@ class Server:
                 r.done_epochs = sum(r.current)
                 if r.done_epochs >= r.epochs:
                     r.completed = end
-                    self._retire(r)
                     finished.append(r)
                     if on_complete:
                         on_complete(r)
-                    if not keep_outputs:
-                        r.out.free()
-                        r.engine.close()
+                    self._retire(r, keep_outputs)
                 else:
                     still.append(r)
             active = still

@@ -21,7 +21,7 @@ def test_requests_in_the_mix_equal_alone(monkeypatch):
     assert len(reqs) >= 4
     server = serve.Server(threads=4)
     outs = {}
-    done = server.serve(reqs, on_complete=lambda r: outs.__setitem__(r.id, r.out.to_numpy()), keep_outputs=True)
+    done = server.serve(reqs, on_complete=lambda r: outs.__setitem__(r.id, r.out.to_numpy(r.out_bytes)), keep_outputs=True)
     assert len(done) == len(reqs)
     assert max(server.batch_sizes) >= 2, "requests must overlap on the device"
     for r in done:
