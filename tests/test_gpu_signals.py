@@ -95,3 +95,18 @@ def test_low_and_round_rates(spe, fmt):
 def test_noise_index_beyond_32_bits():
     sc = Scenario("late_epoch", 61_440, "IQ4", 2, _channels("E5a", 2, 4, 61_440), noise=True, seed=2**63 + 5)
     _check(sc, first=1_000_003, epochs=2)  # absolute sample index ~6.1e10
+
+
+def test_many_channels_take_several_passes():
+    """70 channels: passes of up to 32 channels (4 setup threads each) and a short last pass (8 threads each)."""
+    chans = []
+    for k, sig in enumerate(["L1CA", "E1", "L5", "B1I", "G1", "E5a", "B3I"]):
+        chans += _channels(sig, 10, 90 + k, 61_440)
+    sc = Scenario("many", 61_440, "IQ4", 2, chans, noise=True, seed=31)
+    _check(sc, first=777, epochs=2, gain=0.6)
+
+
+def test_highest_rate():
+    sc = Scenario("rate_200000", 200_000, "IQ8", 2, _channels("L5", 3, 5, 200_000) + _channels("E1", 3, 6, 200_000),
+                  noise=True, seed=8)
+    _check(sc, first=5, epochs=2)
