@@ -424,13 +424,14 @@ template <uint32_t AXIS, bool LEAD>
 __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos, uint64_t step,
                                                 uint32_t vt_addr, float cre, float cim, float wre, float wim,
                                                 uint64_t lead_add, uint32_t lead_bits) {
-    // Eight samples at a time: their entry reads, then their value reads, are in flight together before the
-    // arithmetic that needs them, so LDS latency overlaps instead of adding up sample by sample.
-    constexpr uint32_t B = 8;
+    // Eight samples at a time, software-pipelined over three stages: while batch n is computed, the value reads of
+    // batch n + 1 and the entry reads of batch n + 2 are in flight, so LDS latency hides behind arithmetic.
+    constexpr uint32_t B = 8, NB = STEPS / B;
     const float nwim = -wim;
-#pragma unroll
-    for (uint32_t k0 = 0; k0 < STEPS; k0 += B) {
-        uint32_t va[B];
+    uint32_t va[2][B];
+    float vx[2][B], vy[2][B];
+    // Stage 1: the value-table addresses of the next batch, from its staged entries.
+    auto addresses = [&](uint32_t (&out)[B]) {
         if (LEAD) {
             // chip.zig:143-159 / kernel.zig:233-234: the leading bits come from the entry the lead reaches.
             uint32_t bm[B], bl[B];
@@ -441,43 +442,53 @@ __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[
                 bl[k] = lds_load_u8((uint32_t)((p + lead_add) >> 32));
             }
 #pragma unroll
-            for (uint32_t k = 0; k < B; k++) va[k] = vt_addr + ((bl[k] & lead_bits) | (bm[k] & ~lead_bits)) * 8u;
+            for (uint32_t k = 0; k < B; k++) out[k] = vt_addr + ((bl[k] & lead_bits) | (bm[k] & ~lead_bits)) * 8u;
         } else {
             uint32_t e[B];
 #pragma unroll
             for (uint32_t k = 0; k < B; k++) e[k] = lds_load_u8((uint32_t)((pos + (uint64_t)k * step) >> 32));
 #pragma unroll
-            for (uint32_t k = 0; k < B; k++) va[k] = vt_addr + e[k];
+            for (uint32_t k = 0; k < B; k++) out[k] = vt_addr + e[k];
         }
         pos += (uint64_t)B * step;
-        float vx[B], vy[B];
+    };
+    // Stage 2: the values.
+    auto values = [&](const uint32_t (&addr)[B], float (&x)[B], float (&y)[B]) {
 #pragma unroll
         for (uint32_t k = 0; k < B; k++) {
             if (AXIS == AXIS_BOTH) {
-                const float2 v = lds_load_f2(va[k]);
-                vx[k] = v.x;
-                vy[k] = v.y;
+                const float2 v = lds_load_f2(addr[k]);
+                x[k] = v.x;
+                y[k] = v.y;
             } else if (AXIS == AXIS_RE) {
-                vx[k] = lds_load_f32(va[k]);
+                x[k] = lds_load_f32(addr[k]);
             } else {
-                vy[k] = lds_load_f32(va[k] + 4);
+                y[k] = lds_load_f32(addr[k] + 4);
             }
         }
+    };
+    addresses(va[0]);
+    values(va[0], vx[0], vy[0]);
+    addresses(va[1]);
+#pragma unroll
+    for (uint32_t n = 0; n < NB; n++) {
+        const uint32_t cur = n & 1, nxt = cur ^ 1;
+        if (n + 1 < NB) values(va[nxt], vx[nxt], vy[nxt]);
+        if (n + 2 < NB) addresses(va[cur]);
+        // Stage 3: accumulate (kernel.zig:241-258) and advance the rotor (Rotor.advance, rotor.zig:52-58).
 #pragma unroll
         for (uint32_t k = 0; k < B; k++) {
-            const uint32_t i = k0 + k;
-            // kernel.zig:241-258
+            const uint32_t i = n * B + k;
             if (AXIS == AXIS_BOTH) {
-                ar[i] = fmaf(vx[k], cre, fmaf(-vy[k], cim, ar[i]));
-                ai[i] = fmaf(vx[k], cim, fmaf(vy[k], cre, ai[i]));
+                ar[i] = fmaf(vx[cur][k], cre, fmaf(-vy[cur][k], cim, ar[i]));
+                ai[i] = fmaf(vx[cur][k], cim, fmaf(vy[cur][k], cre, ai[i]));
             } else if (AXIS == AXIS_RE) {
-                ar[i] = fmaf(vx[k], cre, ar[i]);
-                ai[i] = fmaf(vx[k], cim, ai[i]);
+                ar[i] = fmaf(vx[cur][k], cre, ar[i]);
+                ai[i] = fmaf(vx[cur][k], cim, ai[i]);
             } else {
-                ar[i] = fmaf(-vy[k], cim, ar[i]);
-                ai[i] = fmaf(vy[k], cre, ai[i]);
+                ar[i] = fmaf(-vy[cur][k], cim, ar[i]);
+                ai[i] = fmaf(vy[cur][k], cre, ai[i]);
             }
-            // Rotor.advance (rotor.zig:52-58)
             const float t = fmaf(cre, wre, cim * nwim);
             cim = fmaf(cre, wim, cim * wre);
             cre = t;
