@@ -76,6 +76,7 @@ enum : uint32_t {
     ERR_STAGE = 1u,    // a row needs more staged entries than the tables allow (rate outside +-3 %, lead too long)
     ERR_MASKS = 2u,    // a row reads a mask beyond mask_count or the mask buffer
     ERR_TABLE = 4u,    // a row names a table out of range or a zero code rate
+    ERR_LAYOUT = 8u,   // the build placed the value-table slots away from LDS address 0
 };
 
 // Work description for the persistent batched kernel: one segment = one window of one job.

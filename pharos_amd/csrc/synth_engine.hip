@@ -436,7 +436,7 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
     e->view.w_slot = (std::max<uint32_t>(w_slot_max, 16) + 31) / 64 * 64 + 32;
     // Channels per LDS pass: as many as fit next to the rows within LDS_LIMIT, less the reduction and work-queue
     // words (four workgroups per CU); the tile's sums reuse the same bytes afterwards.
-    const uint32_t budget = LDS_LIMIT - 64 - 16 - ROWS_BYTES;
+    const uint32_t budget = LDS_LIMIT - 64 - 16 - (uint32_t)sizeof(g_wave_vt) - ROWS_BYTES;
     const uint32_t per_ch = (uint32_t)sizeof(ChanRec) + VT_BYTES + e->view.w_slot;
     uint32_t chunk = budget / per_ch;
     chunk = std::max<uint32_t>(1, std::min<uint32_t>(chunk, MAX_CHUNK));
@@ -475,6 +475,7 @@ static int take_device_error(pharos_synth_engine *e, hipStream_t stream) {
     set_error("signal parameters are outside the supported calculation range (device flags 0x%x: %s%s%s)", err,
               (err & ERR_STAGE) ? "code rate or lead beyond the staged range; " : "",
               (err & ERR_MASKS) ? "mask bytes missing; " : "", (err & ERR_TABLE) ? "bad table or code rate" : "");
+    if (err & ERR_LAYOUT) set_error("internal: value-table slots are not at LDS address 0 in this build");
     return ST_INTERNAL;
 }
 

@@ -420,10 +420,18 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
 
 // The per-sample loop of one channel for one thread: 32 samples at stride 16 of one rotor block.
 // pos: high word = LDS address of the staged entry byte, low word = Q32.32 fraction of the code position.
-template <uint32_t AXIS, bool LEAD>
+// The value table of the channel each wave is summing, copied to a fixed LDS slot per wave.  It is the kernels'
+// only static LDS variable, so it sits at LDS address 0 (synth_tile checks) and wave w's slot at the literal w * 128:
+// a value read takes the staged byte itself as its address, the slot going in the instruction's offset field, and
+// the per-sample address add disappears.
+__shared__ float2 g_wave_vt[THREADS / 64][16];
+constexpr uint32_t VT_SLOT_BYTES = 16 * sizeof(float2);
+
+template <uint32_t AXIS, bool LEAD, uint32_t WAVE>
 __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos, uint64_t step,
-                                                uint32_t vt_addr, float cre, float cim, float wre, float wim,
-                                                uint64_t lead_add, uint32_t lead_bits) {
+                                                float cre, float cim, float wre, float wim, uint64_t lead_add,
+                                                uint32_t lead_bits) {
+    const uint32_t vt_addr = WAVE * VT_SLOT_BYTES;
     // Eight samples at a time, software-pipelined over three stages: while batch n is computed, the value reads of
     // batch n + 1 and the entry reads of batch n + 2 are in flight, so LDS latency hides behind arithmetic.
     constexpr uint32_t B = 8, NB = STEPS / B;
@@ -493,6 +501,22 @@ __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[
             cim = fmaf(cre, wim, cim * wre);
             cre = t;
         }
+    }
+}
+
+// The loop for a channel's axis and shape, by the wave whose value-table slot it reads.
+template <uint32_t WAVE>
+__device__ __forceinline__ void channel_dispatch(uint32_t axis, bool lead, float (&ar)[STEPS], float (&ai)[STEPS],
+                                                 uint64_t pos, uint64_t step, float cre, float cim, float wre,
+                                                 float wim, uint64_t lead_add, uint32_t lead_bits) {
+    if (lead) {
+        if (axis == AXIS_BOTH) channel_samples<AXIS_BOTH, true, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits);
+        else if (axis == AXIS_RE) channel_samples<AXIS_RE, true, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits);
+        else channel_samples<AXIS_IM, true, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits);
+    } else {
+        if (axis == AXIS_BOTH) channel_samples<AXIS_BOTH, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0);
+        else if (axis == AXIS_RE) channel_samples<AXIS_RE, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0);
+        else channel_samples<AXIS_IM, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0);
     }
 }
 
@@ -635,6 +659,7 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
     const bool block_live = nb < E.spe;
     Lds L = lds_layout(smem, E);
     const uint32_t smem_lds = lds_address(smem);
+    if (tid == 0 && lds_address(g_wave_vt) != 0) atomicOr(err, ERR_LAYOUT);
 
     float ar[STEPS], ai[STEPS];
 #pragma unroll
@@ -672,6 +697,7 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
         const bool fetch = tid < nc_next * row_words;
         if (fetch) next_row = gload_u4(reinterpret_cast<const uint4 *>(A.rows + c1) + tid);
         // The channel sums.
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         if (block_live) {
             for (uint32_t ch = 0; ch < nc; ch++) {
                 const ChanRec &R = L.rec[ch];
@@ -684,17 +710,16 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
                 const uint32_t addr = smem_lds + L.stage_base + ch * E.w_slot + (R.e0 & 3u) + rel;
                 const uint64_t pos = ((uint64_t)addr << 32) | (uint32_t)Pn;
                 const uint64_t step = R.eps_q * LANES16;
-                const uint32_t vt_off = smem_lds + L.vt_base + ch * VT_BYTES;
-                if (R.shape == SHAPE_LEAD) {
-                    const uint64_t lead_add = ((uint64_t)R.lead_whole << 32) + R.lead_q;
-                    if (R.axis == AXIS_BOTH) channel_samples<AXIS_BOTH, true>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
-                    else if (R.axis == AXIS_RE) channel_samples<AXIS_RE, true>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
-                    else channel_samples<AXIS_IM, true>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
-                } else {
-                    if (R.axis == AXIS_BOTH) channel_samples<AXIS_BOTH, false>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
-                    else if (R.axis == AXIS_RE) channel_samples<AXIS_RE, false>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
-                    else channel_samples<AXIS_IM, false>(ar, ai, pos, step, vt_off, cre, cim, R.w_re, R.w_im, 0, 0);
-                }
+                // This channel's values into the wave's slot (lanes 0-31, a dword each); LDS operations of one wave
+                // complete in order, so waiting for the writes is enough before the reads.
+                if ((tid & 63u) < 32u)
+                    reinterpret_cast<uint32_t *>(g_wave_vt[wave])[tid & 31u] =
+                        reinterpret_cast<const uint32_t *>(L.vt + ch * VT_BYTES)[tid & 31u];
+                __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+                const uint64_t lead_add = ((uint64_t)R.lead_whole << 32) + R.lead_q;
+                const bool lead = R.shape == SHAPE_LEAD;
+                if (wave == 0) channel_dispatch<0>(R.axis, lead, ar, ai, pos, step, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
+                else channel_dispatch<1>(R.axis, lead, ar, ai, pos, step, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
             }
         }
         __syncthreads();
