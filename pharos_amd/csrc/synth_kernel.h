@@ -427,11 +427,13 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
 __shared__ float2 g_wave_vt[THREADS / 64][16];
 constexpr uint32_t VT_SLOT_BYTES = 16 * sizeof(float2);
 
+constexpr uint32_t ANY_WAVE = 0xffffffffu;  // one copy for both waves, the slot address in a register
+
 template <uint32_t AXIS, bool LEAD, uint32_t WAVE>
 __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos, uint64_t step,
                                                 float cre, float cim, float wre, float wim, uint64_t lead_add,
-                                                uint32_t lead_bits) {
-    const uint32_t vt_addr = WAVE * VT_SLOT_BYTES;
+                                                uint32_t lead_bits, uint32_t slot) {
+    const uint32_t vt_addr = WAVE == ANY_WAVE ? slot : WAVE * VT_SLOT_BYTES;
     // Eight samples at a time, software-pipelined over three stages: while batch n is computed, the value reads of
     // batch n + 1 and the entry reads of batch n + 2 are in flight, so LDS latency hides behind arithmetic.
     constexpr uint32_t B = 8, NB = STEPS / B;
@@ -504,20 +506,22 @@ __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[
     }
 }
 
-// The loop for a channel's axis and shape, by the wave whose value-table slot it reads.
+// The loop of a plain or TDM channel's axis, by the wave whose value-table slot it reads (a copy per wave).
 template <uint32_t WAVE>
-__device__ __forceinline__ void channel_dispatch(uint32_t axis, bool lead, float (&ar)[STEPS], float (&ai)[STEPS],
-                                                 uint64_t pos, uint64_t step, float cre, float cim, float wre,
-                                                 float wim, uint64_t lead_add, uint32_t lead_bits) {
-    if (lead) {
-        if (axis == AXIS_BOTH) channel_samples<AXIS_BOTH, true, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits);
-        else if (axis == AXIS_RE) channel_samples<AXIS_RE, true, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits);
-        else channel_samples<AXIS_IM, true, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits);
-    } else {
-        if (axis == AXIS_BOTH) channel_samples<AXIS_BOTH, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0);
-        else if (axis == AXIS_RE) channel_samples<AXIS_RE, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0);
-        else channel_samples<AXIS_IM, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0);
-    }
+__device__ __forceinline__ void channel_dispatch(uint32_t axis, float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos,
+                                                 uint64_t step, float cre, float cim, float wre, float wim) {
+    if (axis == AXIS_BOTH) channel_samples<AXIS_BOTH, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0, 0);
+    else if (axis == AXIS_RE) channel_samples<AXIS_RE, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0, 0);
+    else channel_samples<AXIS_IM, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0, 0);
+}
+
+// The loop of a lead channel: rare, so one copy serves both waves.
+__device__ __forceinline__ void lead_dispatch(uint32_t axis, float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos,
+                                              uint64_t step, float cre, float cim, float wre, float wim,
+                                              uint64_t lead_add, uint32_t lead_bits, uint32_t slot) {
+    if (axis == AXIS_BOTH) channel_samples<AXIS_BOTH, true, ANY_WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits, slot);
+    else if (axis == AXIS_RE) channel_samples<AXIS_RE, true, ANY_WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits, slot);
+    else channel_samples<AXIS_IM, true, ANY_WAVE>(ar, ai, pos, step, cre, cim, wre, wim, lead_add, lead_bits, slot);
 }
 
 __device__ __forceinline__ uint32_t bytes_for(uint32_t format, uint32_t samples) {
@@ -716,10 +720,11 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
                     reinterpret_cast<uint32_t *>(g_wave_vt[wave])[tid & 31u] =
                         reinterpret_cast<const uint32_t *>(L.vt + ch * VT_BYTES)[tid & 31u];
                 __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-                const uint64_t lead_add = ((uint64_t)R.lead_whole << 32) + R.lead_q;
-                const bool lead = R.shape == SHAPE_LEAD;
-                if (wave == 0) channel_dispatch<0>(R.axis, lead, ar, ai, pos, step, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
-                else channel_dispatch<1>(R.axis, lead, ar, ai, pos, step, cre, cim, R.w_re, R.w_im, lead_add, R.lead_bits);
+                if (R.shape == SHAPE_LEAD)
+                    lead_dispatch(R.axis, ar, ai, pos, step, cre, cim, R.w_re, R.w_im,
+                                  ((uint64_t)R.lead_whole << 32) + R.lead_q, R.lead_bits, wave * VT_SLOT_BYTES);
+                else if (wave == 0) channel_dispatch<0>(R.axis, ar, ai, pos, step, cre, cim, R.w_re, R.w_im);
+                else channel_dispatch<1>(R.axis, ar, ai, pos, step, cre, cim, R.w_re, R.w_im);
             }
         }
         __syncthreads();
