@@ -413,7 +413,8 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
             d.tdm_periods = t.tdm_periods;
             d.tdm_half = per / 2;
         }
-        w_slot_max = std::max<uint32_t>(w_slot_max, (uint32_t)w_max + 8);  // staging copies whole dwords from e0 & ~3
+        // Staging copies whole 16-byte units from e0 & ~15: up to 15 bytes before the window and 16 after it.
+        w_slot_max = std::max<uint32_t>(w_slot_max, (uint32_t)w_max + 32);
     }
     void *p;
     int st;

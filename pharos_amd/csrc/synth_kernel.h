@@ -224,7 +224,7 @@ __device__ __forceinline__ void divmod32(uint64_t x, uint32_t d, uint64_t &q, ui
 // stages as 8, a zero entry, where the reference would index out of its table.
 __device__ __forceinline__ uint32_t stage_bytes(uint32_t shape, uint32_t m4, uint32_t e0, uint32_t k_lo, uint32_t table,
                                                 uint64_t period0, const DevEngine &E, uint32_t v, uint32_t d) {
-    const uint32_t head = e0 & 3u;
+    const uint32_t head = e0 & 15u;
     uint32_t o = 0;
 #pragma unroll
     for (uint32_t q = 0; q < 4; q++) {
@@ -266,17 +266,23 @@ __device__ __forceinline__ void stage_window(uint32_t *dst, const uint32_t *src_
     const uint32_t m = lead ? m4 : m4 << 3;
     const uint32_t keep = lead ? 0x07070707u : 0x38383838u;
     const uint32_t sh = lead ? 3u : 0u;
-    for (uint32_t d0 = j; d0 < ndw; d0 += 8 * group) {
-        uint32_t v[8];
+    auto map = [&](uint32_t v) { return ((v >> sh) & keep) ^ __builtin_amdgcn_perm(m, m, (v >> 6) & 0x03030303u); };
+    // Whole 16-byte units: the window starts 16-byte aligned in the table and in LDS, the table carries 32 bytes of
+    // padding past its last entry and the slot holds 16 bytes past the window, so the last unit stays in bounds.
+    const uint32_t nq = (ndw + 3) / 4;
+    const gptr<u32x4_t> src4 = (gptr<u32x4_t>)src;
+    u32x4_t *dst4 = reinterpret_cast<u32x4_t *>(dst);
+    for (uint32_t q0 = j; q0 < nq; q0 += 8 * group) {
+        u32x4_t v[8];
 #pragma unroll
         for (uint32_t k = 0; k < 8; k++) {
-            const uint32_t d = d0 + k * group;
-            v[k] = d < ndw ? src[d] : 0u;
+            const uint32_t q = q0 + k * group;
+            v[k] = q < nq ? src4[q] : u32x4_t{0u, 0u, 0u, 0u};
         }
 #pragma unroll
         for (uint32_t k = 0; k < 8; k++) {
-            const uint32_t d = d0 + k * group;
-            if (d < ndw) dst[d] = ((v[k] >> sh) & keep) ^ __builtin_amdgcn_perm(m, m, (v[k] >> 6) & 0x03030303u);
+            const uint32_t q = q0 + k * group;
+            if (q < nq) dst4[q] = u32x4_t{map(v[k].x), map(v[k].y), map(v[k].z), map(v[k].w)};
         }
     }
 }
@@ -367,7 +373,7 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
 #pragma unroll
     for (uint32_t w = 0; w < 4; w++)
         if (w <= w_used) m4 |= (uint32_t)gmem(masks)[(uint64_t)row.mask_offset + k_lo + w] << (8 * w);
-    const uint32_t head = e0 & 3u;
+    const uint32_t head = e0 & 15u;
     const uint32_t *src = reinterpret_cast<const uint32_t *>(ext + (e0 - head));
     const uint32_t ndw = (uint32_t)((head + W + 3) / 4);
     const bool wide = (m4 & 0xf8f8f8f8u) != 0;
@@ -711,7 +717,7 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
                 const float cim = (float)(cc * ps + cs * pc);
                 const uint64_t Pn = R.P0 + (uint64_t)(b * RESEED + l) * R.eps_q;
                 const uint32_t rel = (uint32_t)(Pn >> 32) - (uint32_t)(R.P0 >> 32);
-                const uint32_t addr = smem_lds + L.stage_base + ch * E.w_slot + (R.e0 & 3u) + rel;
+                const uint32_t addr = smem_lds + L.stage_base + ch * E.w_slot + (R.e0 & 15u) + rel;
                 const uint64_t pos = ((uint64_t)addr << 32) | (uint32_t)Pn;
                 const uint64_t step = R.eps_q * LANES16;
                 // This channel's values into the wave's slot (lanes 0-31, a dword each); LDS operations of one wave
