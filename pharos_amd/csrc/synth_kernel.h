@@ -246,7 +246,7 @@ __device__ __forceinline__ uint32_t stage_bytes(uint32_t shape, uint32_t m4, uin
             }
         }
         idx = idx < 8u ? idx : 8u;
-        o |= (shape == SHAPE_LEAD ? idx : 8u * idx) << (8 * q);
+        o |= 8u * idx << (8 * q);
     }
     return o;
 }
@@ -262,11 +262,8 @@ __device__ __forceinline__ void stage_window(uint32_t *dst, const uint32_t *src_
         for (uint32_t d = j; d < ndw; d += group) dst[d] = slow(src[d], d);
         return;
     }
-    const bool lead = shape == SHAPE_LEAD;
-    const uint32_t m = lead ? m4 : m4 << 3;
-    const uint32_t keep = lead ? 0x07070707u : 0x38383838u;
-    const uint32_t sh = lead ? 3u : 0u;
-    auto map = [&](uint32_t v) { return ((v >> sh) & keep) ^ __builtin_amdgcn_perm(m, m, (v >> 6) & 0x03030303u); };
+    const uint32_t m = m4 << 3;
+    auto map = [&](uint32_t v) { return (v & 0x38383838u) ^ __builtin_amdgcn_perm(m, m, (v >> 6) & 0x03030303u); };
     // Whole 16-byte units: the window starts 16-byte aligned in the table and in LDS, the table carries 32 bytes of
     // padding past its last entry and the slot holds 16 bytes past the window, so the last unit stays in bounds.
     const uint32_t nq = (ndw + 3) / 4;
@@ -433,6 +430,14 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
 __shared__ float2 g_wave_vt[THREADS / 64][16];
 constexpr uint32_t VT_SLOT_BYTES = 16 * sizeof(float2);
 
+// (mask & a) | (~mask & b) in one v_bfi_b32 (the compiler splits the expression into three or four operations);
+// the mask is wave-uniform and sits in a scalar register.
+__device__ __forceinline__ uint32_t bit_insert(uint32_t mask, uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "s"(mask), "v"(a), "v"(b));
+    return r;
+}
+
 constexpr uint32_t ANY_WAVE = 0xffffffffu;  // one copy for both waves, the slot address in a register
 
 template <uint32_t AXIS, bool LEAD, uint32_t WAVE>
@@ -440,25 +445,23 @@ __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[
                                                 float cre, float cim, float wre, float wim, uint64_t lead_add,
                                                 uint32_t lead_bits, uint32_t slot) {
     const uint32_t vt_addr = WAVE == ANY_WAVE ? slot : WAVE * VT_SLOT_BYTES;
+    const uint32_t lead_bits8 = __builtin_amdgcn_readfirstlane(lead_bits << 3);
     // Eight samples at a time, software-pipelined over three stages: while batch n is computed, the value reads of
     // batch n + 1 and the entry reads of batch n + 2 are in flight, so LDS latency hides behind arithmetic.
-    constexpr uint32_t B = 8, NB = STEPS / B;
+    constexpr uint32_t B = LEAD ? 4 : 8, NB = STEPS / B;
     const float nwim = -wim;
-    uint32_t va[2][B];
+    uint32_t va[2][B], vl[2][B];
     float vx[2][B], vy[2][B];
-    // Stage 1: the value-table addresses of the next batch, from its staged entries.
-    auto addresses = [&](uint32_t (&out)[B]) {
+    // Stage 1: the staged entries of the next batch (value-table addresses; a lead channel's two entries).
+    auto addresses = [&](uint32_t (&out)[B], uint32_t (&out_lead)[B]) {
         if (LEAD) {
             // chip.zig:143-159 / kernel.zig:233-234: the leading bits come from the entry the lead reaches.
-            uint32_t bm[B], bl[B];
 #pragma unroll
             for (uint32_t k = 0; k < B; k++) {
                 const uint64_t p = pos + (uint64_t)k * step;
-                bm[k] = lds_load_u8((uint32_t)(p >> 32));
-                bl[k] = lds_load_u8((uint32_t)((p + lead_add) >> 32));
+                out[k] = lds_load_u8((uint32_t)(p >> 32));
+                out_lead[k] = lds_load_u8((uint32_t)((p + lead_add) >> 32));
             }
-#pragma unroll
-            for (uint32_t k = 0; k < B; k++) out[k] = vt_addr + ((bl[k] & lead_bits) | (bm[k] & ~lead_bits)) * 8u;
         } else {
             uint32_t e[B];
 #pragma unroll
@@ -468,8 +471,13 @@ __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[
         }
         pos += (uint64_t)B * step;
     };
-    // Stage 2: the values.
-    auto values = [&](const uint32_t (&addr)[B], float (&x)[B], float (&y)[B]) {
+    // Stage 2: the values.  Staged bytes are 8 x index, so a lead channel's leading bits (lead_bits << 3) merge in
+    // with one bit-field insert, done here, a batch after the entry reads were issued.
+    auto values = [&](const uint32_t (&entry)[B], const uint32_t (&entry_lead)[B], float (&x)[B], float (&y)[B]) {
+        uint32_t addr[B];
+#pragma unroll
+        for (uint32_t k = 0; k < B; k++)
+            addr[k] = LEAD ? vt_addr + bit_insert(lead_bits8, entry_lead[k], entry[k]) : entry[k];
 #pragma unroll
         for (uint32_t k = 0; k < B; k++) {
             if (AXIS == AXIS_BOTH) {
@@ -483,14 +491,14 @@ __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[
             }
         }
     };
-    addresses(va[0]);
-    values(va[0], vx[0], vy[0]);
-    addresses(va[1]);
+    addresses(va[0], vl[0]);
+    values(va[0], vl[0], vx[0], vy[0]);
+    addresses(va[1], vl[1]);
 #pragma unroll
     for (uint32_t n = 0; n < NB; n++) {
         const uint32_t cur = n & 1, nxt = cur ^ 1;
-        if (n + 1 < NB) values(va[nxt], vx[nxt], vy[nxt]);
-        if (n + 2 < NB) addresses(va[cur]);
+        if (n + 1 < NB) values(va[nxt], vl[nxt], vx[nxt], vy[nxt]);
+        if (n + 2 < NB) addresses(va[cur], vl[cur]);
         // Stage 3: accumulate (kernel.zig:241-258) and advance the rotor (Rotor.advance, rotor.zig:52-58).
 #pragma unroll
         for (uint32_t k = 0; k < B; k++) {
