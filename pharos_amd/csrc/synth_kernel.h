@@ -718,27 +718,36 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
         const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
         if (block_live) {
             for (uint32_t ch = 0; ch < nc; ch++) {
+                // Every field of the channel's record and its value-table dword are read in one LDS round trip,
+                // ahead of the first branch on them.
                 const ChanRec &R = L.rec[ch];
-                if (!R.active) continue;
+                const uint32_t vt_word = reinterpret_cast<const uint32_t *>(L.vt + ch * VT_BYTES)[tid & 31u];
                 const double pc = R.pw_c[l], ps = R.pw_s[l], cc = R.c0_c[b], cs = R.c0_s[b];
+                const uint64_t P0 = R.P0, eps_q = R.eps_q;
+                const uint32_t e0 = R.e0, lead_whole = R.lead_whole, lead_q = R.lead_q, lead_bits = R.lead_bits;
+                const float w_re = R.w_re, w_im = R.w_im;
+                const uint32_t active = R.active, shape = R.shape, axis = R.axis;
+                // This channel's values into the wave's slot (a dword per lane; lanes l and l + 32 write the same
+                // word with the same value): the LDS runs one wave's operations in order, so the value reads that
+                // follow see the writes.
+                reinterpret_cast<uint32_t *>(g_wave_vt[wave])[tid & 31u] = vt_word;
+                // The record's values are taken here, ahead of the test of `active`, so that their loads are not
+                // sunk past that branch into a second round trip (the statement emits no instruction).
+                asm volatile("" ::"v"(pc), "v"(ps), "v"(cc), "v"(cs), "v"(P0), "v"(eps_q), "v"(e0), "v"(lead_whole),
+                             "v"(lead_q), "v"(lead_bits), "v"(w_re), "v"(w_im));
+                if (!active) continue;
                 const float cre = (float)(cc * pc - cs * ps);
                 const float cim = (float)(cc * ps + cs * pc);
-                const uint64_t Pn = R.P0 + (uint64_t)(b * RESEED + l) * R.eps_q;
-                const uint32_t rel = (uint32_t)(Pn >> 32) - (uint32_t)(R.P0 >> 32);
-                const uint32_t addr = smem_lds + L.stage_base + ch * E.w_slot + (R.e0 & 15u) + rel;
+                const uint64_t Pn = P0 + (uint64_t)(b * RESEED + l) * eps_q;
+                const uint32_t rel = (uint32_t)(Pn >> 32) - (uint32_t)(P0 >> 32);
+                const uint32_t addr = smem_lds + L.stage_base + ch * E.w_slot + (e0 & 15u) + rel;
                 const uint64_t pos = ((uint64_t)addr << 32) | (uint32_t)Pn;
-                const uint64_t step = R.eps_q * LANES16;
-                // This channel's values into the wave's slot (lanes 0-31, a dword each); LDS operations of one wave
-                // complete in order, so waiting for the writes is enough before the reads.
-                if ((tid & 63u) < 32u)
-                    reinterpret_cast<uint32_t *>(g_wave_vt[wave])[tid & 31u] =
-                        reinterpret_cast<const uint32_t *>(L.vt + ch * VT_BYTES)[tid & 31u];
-                __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-                if (R.shape == SHAPE_LEAD)
-                    lead_dispatch(R.axis, ar, ai, pos, step, cre, cim, R.w_re, R.w_im,
-                                  ((uint64_t)R.lead_whole << 32) + R.lead_q, R.lead_bits, wave * VT_SLOT_BYTES);
-                else if (wave == 0) channel_dispatch<0>(R.axis, ar, ai, pos, step, cre, cim, R.w_re, R.w_im);
-                else channel_dispatch<1>(R.axis, ar, ai, pos, step, cre, cim, R.w_re, R.w_im);
+                const uint64_t step = eps_q * LANES16;
+                if (shape == SHAPE_LEAD)
+                    lead_dispatch(axis, ar, ai, pos, step, cre, cim, w_re, w_im,
+                                  ((uint64_t)lead_whole << 32) + lead_q, lead_bits, wave * VT_SLOT_BYTES);
+                else if (wave == 0) channel_dispatch<0>(axis, ar, ai, pos, step, cre, cim, w_re, w_im);
+                else channel_dispatch<1>(axis, ar, ai, pos, step, cre, cim, w_re, w_im);
             }
         }
         __syncthreads();
