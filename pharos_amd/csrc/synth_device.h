@@ -56,16 +56,17 @@ struct DevEngine {
 // Per channel, per tile (LDS).
 struct ChanRec {
     double pw_c[LANES16], pw_s[LANES16];  // w1^l, the rotor's f64 lane powers (rotor.zig:25-30)
+    uint64_t pl[LANES16];      // l * eps_q: lane l's code-position offset from its block's first sample
     double c0_c[TILE_BLOCKS], c0_s[TILE_BLOCKS];  // sincos64 of each block's start phase (rotor.zig:37-38)
-    const uint32_t *src;       // the staged window's first dword in the extended table (4-byte aligned)
-    uint64_t P0;               // Q32.32 code position of the tile's first sample, less `first`
-    uint64_t eps_q;
+    // Block b's first-sample code position in the per-sample loop's form: the low word is the Q32.32 fraction; the
+    // high word is the LDS address of the staged byte of that sample's entry (the channel's stage base + the
+    // entry's offset in the staged window), so a lane's position is pb[b] + pl[l].
+    uint64_t pb[TILE_BLOCKS];
+    uint64_t step;             // 16 * eps_q: code advance between a lane's samples
     float w_re, w_im;          // f32 sincos64(16 step) (rotor.zig:31-33)
-    uint32_t active, shape, axis, table;
-    uint32_t e0, k_lo, W, lead_q;
-    uint32_t lead_whole, lead_bits, ndw, m4;  // staged dwords; symbol masks of period offsets 0..3, a byte each
-    uint32_t wide_mask;        // some mask byte exceeds 7 (a value index past any table: staged as the zero entry)
-    uint32_t pad[3];
+    uint32_t active, shape, axis;
+    uint32_t lead_q, lead_whole, lead_bits;
+    uint32_t pad[2];
 };
 static_assert(sizeof(ChanRec) % 16 == 0, "ChanRec keeps 16-byte alignment");
 static_assert(THREADS == MAX_CHUNK * GROUP && TILE_BLOCKS % GROUP == 0 && LANES16 % GROUP == 0,

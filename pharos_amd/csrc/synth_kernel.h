@@ -294,7 +294,8 @@ __device__ __forceinline__ void stage_window(uint32_t *dst, const uint32_t *src_
 // and thread 0 the record's scalars.  A pass of up to THREADS / 8 channels gives each channel 8 threads, a larger
 // one 4.
 template <uint32_t G>
-__device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *stage, const pharos_synth_row &row,
+__device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *stage, uint32_t stage_addr,
+                                           const pharos_synth_row &row,
                                            const DevEngine &E, uint32_t n0, uint32_t tile_len, const uint8_t *masks,
                                            uint64_t mask_len, uint32_t *err, uint32_t j) {
     const bool first_lane = j == 0;
@@ -399,19 +400,23 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
             R.pw_s[l] = as;
         }
     }
+    // Code positions: lane offsets l * eps_q and block starts P0 + 512 b eps_q by repeated addition; the block
+    // starts carry the staged byte's LDS address in their high word (entry offset rel = (P >> 32) - (P0 >> 32)
+    // from the window's first entry, which sits at byte e0 & 15 of the slot).
+    {
+        uint64_t pl = (uint64_t)k0 * eps_q;
+#pragma unroll
+        for (uint32_t i = 0; i < PER; i++, pl += eps_q) R.pl[k0 + i] = pl;
+        const uint32_t hi_base = stage_addr + (e0 & 15u) - (uint32_t)(P0 >> 32);
+        const uint64_t block = (uint64_t)RESEED * eps_q;
+        uint64_t pb = P0 + ((uint64_t)hi_base << 32) + (uint64_t)j * block;
+#pragma unroll
+        for (uint32_t bb = j; bb < TILE_BLOCKS; bb += G, pb += (uint64_t)G * block) R.pb[bb] = pb;
+    }
     if (first_lane) {
-        R.src = src;
-        R.ndw = ndw;
-        R.m4 = m4;
-        R.wide_mask = wide;
         R.shape = shape;
         R.axis = axis;
-        R.table = row.table;
-        R.P0 = P0;
-        R.eps_q = eps_q;
-        R.e0 = e0;
-        R.k_lo = (uint32_t)k_lo;
-        R.W = (uint32_t)W;
+        R.step = eps_q * LANES16;
         R.lead_q = lead_q;
         R.lead_whole = lead_whole;
         R.lead_bits = row.lead_bits;
@@ -698,14 +703,16 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
             const uint32_t c = tid / 8, j = tid % 8;
             if (c < nc)
                 chan_group<8>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
-                              reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot), L.rows[c], E, n0, tile_len, A.masks,
-                              A.mask_len, err, j);
+                              reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot),
+                              smem_lds + L.stage_base + c * E.w_slot, L.rows[c], E, n0, tile_len, A.masks, A.mask_len,
+                              err, j);
         } else {
             const uint32_t c = tid / GROUP, j = tid % GROUP;
             if (c < nc)
                 chan_group<GROUP>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
-                                  reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot), L.rows[c], E, n0, tile_len,
-                                  A.masks, A.mask_len, err, j);
+                                  reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot),
+                                  smem_lds + L.stage_base + c * E.w_slot, L.rows[c], E, n0, tile_len, A.masks,
+                                  A.mask_len, err, j);
         }
         __syncthreads();
         // The next chunk's rows travel from global memory while this chunk computes.
@@ -723,8 +730,8 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
                 const ChanRec &R = L.rec[ch];
                 const uint32_t vt_word = reinterpret_cast<const uint32_t *>(L.vt + ch * VT_BYTES)[tid & 31u];
                 const double pc = R.pw_c[l], ps = R.pw_s[l], cc = R.c0_c[b], cs = R.c0_s[b];
-                const uint64_t P0 = R.P0, eps_q = R.eps_q;
-                const uint32_t e0 = R.e0, lead_whole = R.lead_whole, lead_q = R.lead_q, lead_bits = R.lead_bits;
+                const uint64_t pl = R.pl[l], pb = R.pb[b], step = R.step;
+                const uint32_t lead_whole = R.lead_whole, lead_q = R.lead_q, lead_bits = R.lead_bits;
                 const float w_re = R.w_re, w_im = R.w_im;
                 const uint32_t active = R.active, shape = R.shape, axis = R.axis;
                 // This channel's values into the wave's slot (a dword per lane; lanes l and l + 32 write the same
@@ -733,16 +740,12 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
                 reinterpret_cast<uint32_t *>(g_wave_vt[wave])[tid & 31u] = vt_word;
                 // The record's values are taken here, ahead of the test of `active`, so that their loads are not
                 // sunk past that branch into a second round trip (the statement emits no instruction).
-                asm volatile("" ::"v"(pc), "v"(ps), "v"(cc), "v"(cs), "v"(P0), "v"(eps_q), "v"(e0), "v"(lead_whole),
+                asm volatile("" ::"v"(pc), "v"(ps), "v"(cc), "v"(cs), "v"(pl), "v"(pb), "v"(step), "v"(lead_whole),
                              "v"(lead_q), "v"(lead_bits), "v"(w_re), "v"(w_im));
                 if (!active) continue;
                 const float cre = (float)(cc * pc - cs * ps);
                 const float cim = (float)(cc * ps + cs * pc);
-                const uint64_t Pn = P0 + (uint64_t)(b * RESEED + l) * eps_q;
-                const uint32_t rel = (uint32_t)(Pn >> 32) - (uint32_t)(P0 >> 32);
-                const uint32_t addr = smem_lds + L.stage_base + ch * E.w_slot + (e0 & 15u) + rel;
-                const uint64_t pos = ((uint64_t)addr << 32) | (uint32_t)Pn;
-                const uint64_t step = eps_q * LANES16;
+                const uint64_t pos = pb + pl;
                 if (shape == SHAPE_LEAD)
                     lead_dispatch(axis, ar, ai, pos, step, cre, cim, w_re, w_im,
                                   ((uint64_t)lead_whole << 32) + lead_q, lead_bits, wave * VT_SLOT_BYTES);
