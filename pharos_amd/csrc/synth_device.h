@@ -24,7 +24,7 @@ constexpr uint32_t VT_BYTES = 128;   // value table: 16 x (re, im) f32, entries 
 enum Shape : uint32_t { SHAPE_PLAIN = 0, SHAPE_TDM = 1, SHAPE_LEAD = 2 };
 enum Axis : uint32_t { AXIS_BOTH = 0, AXIS_RE = 1, AXIS_IM = 2 };
 
-// A code table on the device.  ext[i] = 8 * states[i % per] + 64 * (i / per) for i < ext_len: one period plus
+// A code table on the device.  ext[i] = 8 * states[i % per] + i / per for i < ext_len: one period plus
 // the entries a tile can run past its end, each byte naming its state and its period offset (0..3).  Staging a
 // tile turns each byte into 8 x its value index (the state with its period's symbol mask applied), the byte offset
 // of the value in the channel's value table.

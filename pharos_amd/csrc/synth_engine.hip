@@ -385,7 +385,7 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
         }
         const uint32_t ext_len = per + (uint32_t)w_max;
         std::vector<uint8_t> ext(ext_len + 32, 0);
-        for (uint32_t i = 0; i < ext_len; i++) ext[i] = (uint8_t)(8u * t.states[i % per] + 64u * (i / per));
+        for (uint32_t i = 0; i < ext_len; i++) ext[i] = (uint8_t)(8u * t.states[i % per] + i / per);
         DevTable &d = dt[k];
         memset(&d, 0, sizeof d);
         void *p;

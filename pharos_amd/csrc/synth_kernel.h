@@ -73,6 +73,18 @@ __device__ __forceinline__ float log_poly(float x) {
     return fmaf(e, 0.693359375f, r);
 }
 
+// sqrtf of a noise radius, correctly rounded as the library's, without its scaling of inputs below 2^-96.  A radius
+// 44.36 - 2 log(v) is 0 or at least 2^-18: below 12 it is the exact difference of two multiples of 2^-18.
+// v_sqrt_f32 is within one ulp; the residuals of its two neighbours pick the correctly rounded value, the same
+// steps the library takes for such inputs.
+__device__ __forceinline__ float sqrt_radius(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u), up = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = fmaf(-dn, s, x) <= 0.0f ? dn : s;
+    r = fmaf(-up, s, x) > 0.0f ? up : r;
+    return r;
+}
+
 __device__ __forceinline__ void sincos_turn(uint32_t phase, float &c, float &s) {
     float x = (float)(phase & 0x3fffffffu) * 0x1.921fb6p-30f;
     float x2 = x * x;
@@ -111,7 +123,7 @@ __device__ __forceinline__ void noise_sample(uint64_t seed, uint64_t a, float &r
     }
     float n2 = fmaf(log_poly(v), -2.0f, 0x1.62e43p+5f);
     n2 = n2 > 0.0f ? n2 : 0.0f;
-    float r = sqrtf(n2);
+    float r = sqrt_radius(n2);
     float c, s;
     sincos_turn(ang, c, s);
     re = r * c;
@@ -146,7 +158,7 @@ __device__ __forceinline__ void noise_group4(uint64_t seed, uint64_t g, float (&
     for (int l = 0; l < 4; l++) {
         float n2 = fmaf(log_poly(v[l]), -2.0f, 0x1.62e43p+5f);
         n2 = n2 > 0.0f ? n2 : 0.0f;
-        const float r = sqrtf(n2);
+        const float r = sqrt_radius(n2);
         float c, s;
         sincos_turn(ang[l], c, s);
         re[l] = r * c;
@@ -229,7 +241,7 @@ __device__ __forceinline__ uint32_t stage_bytes(uint32_t shape, uint32_t m4, uin
 #pragma unroll
     for (uint32_t q = 0; q < 4; q++) {
         const uint32_t x = (v >> (8 * q)) & 0xffu;
-        const uint32_t w = x >> 6, st = (x >> 3) & 7u;
+        const uint32_t w = x & 3u, st = (x >> 3) & 7u;
         const uint32_t m = (m4 >> (8 * w)) & 0xffu;
         uint32_t idx = st ^ m;
         if (shape == SHAPE_TDM) {
@@ -263,7 +275,7 @@ __device__ __forceinline__ void stage_window(uint32_t *dst, const uint32_t *src_
         return;
     }
     const uint32_t m = m4 << 3;
-    auto map = [&](uint32_t v) { return (v & 0x38383838u) ^ __builtin_amdgcn_perm(m, m, (v >> 6) & 0x03030303u); };
+    auto map = [&](uint32_t v) { return (v & 0x38383838u) ^ __builtin_amdgcn_perm(m, m, v & 0x03030303u); };
     // Whole 16-byte units: the window starts 16-byte aligned in the table and in LDS, the table carries 32 bytes of
     // padding past its last entry and the slot holds 16 bytes past the window, so the last unit stays in bounds.
     const uint32_t nq = (ndw + 3) / 4;
