@@ -326,6 +326,29 @@ static void build_turn(std::vector<double> &t) {
     }
 }
 
+// The turn table is the same for every engine: one copy per device, made on first use and kept for the life of the
+// process (64 KB).
+static std::mutex g_turn_mu;
+static const double *g_turn[256];
+static int shared_turn_table(int device, const double **out) {
+    if (device < 0 || device >= 256) { set_error("device %d out of range", device); return ST_ARG; }
+    std::lock_guard<std::mutex> lk(g_turn_mu);
+    if (!g_turn[device]) {
+        std::vector<double> turn;
+        build_turn(turn);
+        void *p = nullptr;
+        if (hipMalloc(&p, turn.size() * sizeof(double)) != hipSuccess) { set_error("turn table allocation failed"); return ST_NOMEM; }
+        if (hipMemcpy(p, turn.data(), turn.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(p);
+            set_error("turn table upload failed");
+            return ST_INTERNAL;
+        }
+        g_turn[device] = (const double *)p;
+    }
+    *out = g_turn[device];
+    return ST_OK;
+}
+
 extern "C" int pharos_synth_create(const pharos_synth_config *config, const pharos_synth_table *tables, uint32_t table_count,
                                    int device, pharos_synth_engine **out) {
     g_error[0] = 0;
@@ -354,7 +377,13 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
     const uint32_t spe = c.samples_per_epoch;
     const uint32_t t_eff = std::min<uint32_t>(TILE, (spe + RESEED - 1) / RESEED * RESEED);
 
+    // Everything the engine reads from device memory goes in one allocation, uploaded with one copy: per table its
+    // extended entries (plus 32 zero bytes) and TDM pilot, then the table descriptors, the engine record and the
+    // error word.  Creating an engine takes one device allocation however many tables it has.
+    auto align = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
     std::vector<DevTable> dt(table_count);
+    std::vector<uint64_t> ext_off(table_count), pilot_off(table_count, 0);
+    uint64_t total = 0;
     uint32_t w_slot_max = 0;
     for (uint32_t k = 0; k < table_count; k++) {
         const pharos_synth_table &t = tables[k];
@@ -383,18 +412,10 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
             engine_free(e);
             return ST_ARG;
         }
-        const uint32_t ext_len = per + (uint32_t)w_max;
-        std::vector<uint8_t> ext(ext_len + 32, 0);
-        for (uint32_t i = 0; i < ext_len; i++) ext[i] = (uint8_t)(8u * t.states[i % per] + i / per);
         DevTable &d = dt[k];
         memset(&d, 0, sizeof d);
-        void *p;
-        int st;
-        if ((st = dev_alloc(e, &p, ext.size()))) { engine_free(e); return st; }
-        if (hipMemcpy(p, ext.data(), ext.size(), hipMemcpyHostToDevice) != hipSuccess) { set_error("table upload failed"); engine_free(e); return ST_INTERNAL; }
-        d.ext = (const uint8_t *)p;
         d.per = per;
-        d.ext_len = ext_len;
+        d.ext_len = per + (uint32_t)w_max;
         d.count = t.value_count;
         bool any_re = false, any_im = false;
         for (int v = 0; v < 8; v++) {
@@ -405,10 +426,11 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
         }
         d.axis = !any_im ? AXIS_RE : (!any_re ? AXIS_IM : AXIS_BOTH);  // code_tables.zig:89-99
         d.w_max = (uint32_t)w_max;
+        ext_off[k] = total;
+        total = align(total + d.ext_len + 32);
         if (t.tdm_periods) {
-            if ((st = dev_alloc(e, &p, t.tdm_pilot_len))) { engine_free(e); return st; }
-            if (hipMemcpy(p, t.tdm_pilot, t.tdm_pilot_len, hipMemcpyHostToDevice) != hipSuccess) { set_error("pilot upload failed"); engine_free(e); return ST_INTERNAL; }
-            d.pilot = (const uint8_t *)p;
+            pilot_off[k] = total;
+            total = align(total + t.tdm_pilot_len);
             d.pilot_len = t.tdm_pilot_len;
             d.tdm_periods = t.tdm_periods;
             d.tdm_half = per / 2;
@@ -416,16 +438,33 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
         // Staging copies whole 16-byte units from e0 & ~15: up to 15 bytes before the window and 16 after it.
         w_slot_max = std::max<uint32_t>(w_slot_max, (uint32_t)w_max + 32);
     }
-    void *p;
+    const uint64_t tables_off = total;
+    total = align(total + sizeof(DevTable) * std::max<uint32_t>(table_count, 1));
+    const uint64_t engine_off = total;
+    total = align(total + sizeof(DevEngine));
+    const uint64_t err_off = total;
+    total += 64;
+    const double *turn = nullptr;
     int st;
-    if ((st = dev_alloc(e, &p, sizeof(DevTable) * std::max<uint32_t>(table_count, 1)))) { engine_free(e); return st; }
-    if (table_count && hipMemcpy(p, dt.data(), sizeof(DevTable) * table_count, hipMemcpyHostToDevice) != hipSuccess) { set_error("table upload failed"); engine_free(e); return ST_INTERNAL; }
-    e->view.tables = (const DevTable *)p;
-    std::vector<double> turn;
-    build_turn(turn);
-    if ((st = dev_alloc(e, &p, turn.size() * sizeof(double)))) { engine_free(e); return st; }
-    if (hipMemcpy(p, turn.data(), turn.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) { set_error("turn table upload failed"); engine_free(e); return ST_INTERNAL; }
-    e->view.turn = (const double *)p;
+    if ((st = shared_turn_table(device, &turn))) { engine_free(e); return st; }
+    void *p;
+    if ((st = dev_alloc(e, &p, total))) { engine_free(e); return st; }
+    uint8_t *base = (uint8_t *)p;
+    std::vector<uint8_t> blob(total, 0);
+    for (uint32_t k = 0; k < table_count; k++) {
+        const pharos_synth_table &t = tables[k];
+        DevTable &d = dt[k];
+        uint8_t *ext = blob.data() + ext_off[k];
+        for (uint32_t i = 0; i < d.ext_len; i++) ext[i] = (uint8_t)(8u * t.states[i % d.per] + i / d.per);
+        d.ext = base + ext_off[k];
+        if (t.tdm_periods) {
+            memcpy(blob.data() + pilot_off[k], t.tdm_pilot, t.tdm_pilot_len);
+            d.pilot = base + pilot_off[k];
+        }
+    }
+    if (table_count) memcpy(blob.data() + tables_off, dt.data(), sizeof(DevTable) * table_count);
+    e->view.tables = (const DevTable *)(base + tables_off);
+    e->view.turn = turn;
     e->view.seed = c.seed;
     e->view.bpe = bytes_for_host(c.format, spe);
     e->view.ntables = table_count;
@@ -451,12 +490,14 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
         engine_free(e);
         return ST_ARG;
     }
-    if ((st = dev_alloc(e, &p, sizeof(DevEngine)))) { engine_free(e); return st; }
-    e->d_engine = (DevEngine *)p;
-    if (hipMemcpy(p, &e->view, sizeof(DevEngine), hipMemcpyHostToDevice) != hipSuccess) { set_error("engine upload failed"); engine_free(e); return ST_INTERNAL; }
-    if ((st = dev_alloc(e, &p, 64))) { engine_free(e); return st; }
-    e->d_err = (uint32_t *)p;
-    if (hipMemset(p, 0, 64) != hipSuccess) { set_error("hipMemset failed"); engine_free(e); return ST_INTERNAL; }
+    memcpy(blob.data() + engine_off, &e->view, sizeof(DevEngine));
+    if (hipMemcpy(base, blob.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("engine upload failed");
+        engine_free(e);
+        return ST_INTERNAL;
+    }
+    e->d_engine = (DevEngine *)(base + engine_off);
+    e->d_err = (uint32_t *)(base + err_off);
     *out = e;
     return ST_OK;
 }

@@ -1,7 +1,8 @@
 """Continuous-batching request loop over the batched engine (one GPU).
 
 Requests are runs (scenarios) arriving over time. The loop works in iterations:
-1. It admits every arrived request whose device memory fits the free HBM (pharos_synth_mem_info), oldest first.
+1. It admits every arrived request whose buffers fit the device memory left (free HBM less a reserve when the loop
+   starts, carved by an arena), oldest first.
 2. It submits one persistent batch holding the next slice of every admitted request. A slice is a whole number
    of 100-epoch AGC windows, sized so it carries about SLICE_WORK channel-samples.
 3. While the GPU runs that batch, worker threads derive each request's following slice and copy it into the
@@ -13,6 +14,7 @@ whatever else shares its batches. Latency is measured from a request's arrival t
 writes its last epoch. Its output is in device memory when `on_complete` is called with it, and is freed after
 that unless the loop is asked to keep outputs.
 """
+import bisect
 import collections
 import ctypes as C
 import os
@@ -68,10 +70,10 @@ class Request:
         masks = int(L.lib().pharos_scenario_mask_bound(C_.byref(s), chans, self.slice_epochs)) + 16
         return self.slice_epochs * max(1, len(self.scenario.channels)) * 64, masks
 
-    def device_need(self):
+    def buffer_sizes(self):
+        """Bytes of the request's device buffers: its output, then two (rows, masks) pairs."""
         rows, masks = self.slice_bytes()
-        c = BufferPool._cls
-        return c(self.out_bytes) + 2 * (c(rows) + c(masks)) + (8 << 20)
+        return [self.out_bytes, rows, masks, rows, masks]
 
     def gain_mode(self):
         return 1 if self.scenario.gain_db is not None else 0
@@ -117,37 +119,103 @@ def _next_slice(req, first, buffers):
     return _copy_into(buffers, _prepare(req, first))
 
 
-class BufferPool:
-    """Device buffers kept for reuse by size class (powers of two), so admissions and retirements do not allocate
-    and free device memory each time."""
+class ArenaBuffer:
+    """A block of an arena: .ptr, .nbytes, .at() and .to_numpy() as a DeviceBuffer has; free() returns it."""
+    __slots__ = ("arena", "block", "offset", "nbytes", "ptr")
 
-    def __init__(self, device, keep_bytes=32 << 30):
+    def __init__(self, arena, block, offset, nbytes):
+        self.arena, self.block, self.offset, self.nbytes = arena, block, offset, nbytes
+        self.ptr = arena.blocks[block].at(offset)
+
+    def at(self, offset):
+        return self.ptr + int(offset)
+
+    def to_numpy(self, nbytes=None, offset=0):
+        n = self.nbytes - offset if nbytes is None else int(nbytes)
+        out = np.empty(n, dtype=np.uint8)
+        if n:
+            check(_lib.lib().pharos_synth_memcpy(out.ctypes.data, self.at(offset), n, 1))
+        return out
+
+    def free(self):
+        if self.ptr is not None:
+            self.arena.release(self)
+            self.ptr = None
+
+
+class DeviceArena:
+    """Device memory for the loop's buffers, carved from a few large blocks: blocks of `block_bytes` (or one
+    buffer's size, if larger) are allocated from the driver as needed, up to `limit_bytes` in all, and are only
+    returned at close().  Buffers are placed first fit in a block's free ranges, which merge with their neighbours
+    when freed, so admissions and retirements make no driver calls in steady state (a large device allocation or
+    free takes milliseconds)."""
+    ALIGN = 4096
+
+    def __init__(self, device, limit_bytes, block_bytes=16 << 30):
         self.device = device
-        self.keep = keep_bytes
-        self.free = collections.defaultdict(list)
-        self.held = 0
+        self.limit = int(limit_bytes)
+        self.block_bytes = int(block_bytes)
+        self.blocks = []        # DeviceBuffer per block
+        self.ranges = []        # per block: sorted [offset, size] free ranges
+        self.used = 0
 
-    @staticmethod
-    def _cls(n):
-        return 1 << max(16, (max(int(n), 1) - 1).bit_length())
+    def _round(self, n):
+        return (max(int(n), 1) + self.ALIGN - 1) // self.ALIGN * self.ALIGN
 
-    def take(self, n):
-        c = self._cls(n)
-        if self.free[c]:
-            self.held -= c
-            return self.free[c].pop()
-        return DeviceBuffer(c, self.device)
+    def reserved(self):
+        return sum(b.nbytes for b in self.blocks)
 
-    def give(self, buf):
-        c = buf.nbytes
-        if self.held + c <= self.keep:
-            self.free[c].append(buf)
-            self.held += c
+    def alloc(self, nbytes):
+        """An ArenaBuffer of at least nbytes, or None if it does not fit within the limit."""
+        n = self._round(nbytes)
+        for bi, free in enumerate(self.ranges):
+            for k, (off, size) in enumerate(free):
+                if size >= n:
+                    if size == n:
+                        del free[k]
+                    else:
+                        free[k] = [off + n, size - n]
+                    self.used += n
+                    return ArenaBuffer(self, bi, off, n)
+        size = max(self.block_bytes, n)
+        if self.reserved() + size > self.limit:
+            size = n
+            if self.reserved() + size > self.limit:
+                return None
+        self.blocks.append(DeviceBuffer(size, self.device))
+        self.ranges.append([[n, size - n]] if size > n else [])
+        self.used += n
+        return ArenaBuffer(self, len(self.blocks) - 1, 0, n)
+
+    def alloc_all(self, sizes):
+        """Buffers for every size, or None (and nothing held) if they do not all fit."""
+        got = []
+        for n in sizes:
+            b = self.alloc(n)
+            if b is None:
+                for g in got:
+                    g.free()
+                return None
+            got.append(b)
+        return got
+
+    def release(self, buf):
+        free = self.ranges[buf.block]
+        off, n = buf.offset, buf.nbytes
+        k = bisect.bisect_left(free, [off, 0])
+        if k < len(free) and free[k][0] == off + n:       # merge with the following range
+            n += free[k][1]
+            del free[k]
+        if k > 0 and free[k - 1][0] + free[k - 1][1] == off:  # ... and the preceding one
+            free[k - 1][1] += n
         else:
-            buf.free()
+            free.insert(k, [off, n])
+        self.used -= buf.nbytes
 
-    def pooled(self):
-        return self.held
+    def close(self):
+        for b in self.blocks:
+            b.free()
+        self.blocks, self.ranges, self.used = [], [], 0
 
 
 class Server:
@@ -156,26 +224,30 @@ class Server:
         self.device = device
         self.reserve = reserve
         self.tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, (C.c_uint32 * 2)())
-        self.pool = ThreadPoolExecutor(max_workers=threads or min(16, os.cpu_count() or 4))
+        threads = threads or min(16, os.cpu_count() or 4)
+        # Two pools: the next slices of admitted requests, which the next batch waits for, never queue behind the
+        # work of requests that just arrived (their engines and first slices).
+        self.pool = ThreadPoolExecutor(max_workers=threads)
+        self.arrivals = ThreadPoolExecutor(max_workers=max(1, threads // 4))
         self.stream = Stream(device)
-        self.buffers = BufferPool(device)
+        free, total = C.c_uint64(), C.c_uint64()
+        check(_lib.lib().pharos_synth_mem_info(device, C.byref(free), C.byref(total)))
+        self.arena = DeviceArena(device, max(0, int(free.value) - reserve))
         self.iterations = 0
         self.batch_sizes = []
         self.phase_s = collections.Counter()  # host seconds per loop phase
 
-    def _free_hbm(self):
-        """Free device memory, counting what the buffer pool holds as free."""
-        free, total = C.c_uint64(), C.c_uint64()
-        check(_lib.lib().pharos_synth_mem_info(self.device, C.byref(free), C.byref(total)))
-        return int(free.value) + self.buffers.pooled()
-
     def _admit(self, req):
-        req.out = self.buffers.take(req.out_bytes)
-        rows, masks = req.slice_bytes()
+        """Places the request's buffers in the arena; False (and nothing held) if they do not fit now."""
+        got = self.arena.alloc_all(req.buffer_sizes())
+        if got is None:
+            return False
+        req.out = got[0]
         # Two row/mask buffers per request: the next slice is written into one while the batch reads the other.
-        req.buffers = [(self.buffers.take(rows), self.buffers.take(masks)) for _ in range(2)]
+        req.buffers = [(got[1], got[2]), (got[3], got[4])]
         req.flip = 0
         req.admitted = time.perf_counter()
+        return True
 
     def _spare(self, req):
         """The row/mask buffers the next slice goes into (not the ones the running batch reads)."""
@@ -186,13 +258,17 @@ class Server:
     def _retire(self, req, keep_output):
         for pair in req.buffers:
             for b in pair:
-                self.buffers.give(b)
+                b.free()
         req.buffers = None
         req.staged = None
         if not keep_output:
-            self.buffers.give(req.out)
+            req.out.free()
             req.out = None
             req.engine.close()
+
+    def close(self):
+        """Returns the arena's device memory (outputs kept by serve(keep_outputs=True) included)."""
+        self.arena.close()
 
     def _job(self, req):
         first, n, d_rows, d_masks, mask_len = req.staged
@@ -218,18 +294,17 @@ class Server:
             now = time.perf_counter()
             while pending and pending[0].arrival_abs <= now:
                 req = pending.popleft()
-                req.arrived = self.pool.submit(_arrive, req, self.device)
+                req.arrived = self.arrivals.submit(_arrive, req, self.device)
                 waiting.append(req)
-            # Admission by free device memory, oldest first, of requests whose first slice is ready.
+            # Admission by device memory, oldest first, of requests whose first slice is ready.
             admitted = []
             ta = time.perf_counter()
             while waiting and (waiting[0].arrived.done() or not active):
-                need = waiting[0].device_need()
-                if self._free_hbm() - self.reserve < need and active:
+                if not self._admit(waiting[0]):
+                    if not active and not admitted:
+                        raise RuntimeError(f"request {waiting[0].id} needs more device memory than the loop has")
                     break
-                req = waiting.popleft()
-                self._admit(req)
-                admitted.append(req)
+                admitted.append(waiting.popleft())
             tb = time.perf_counter()
             for req in admitted:
                 prepared = req.arrived.result()
@@ -305,6 +380,7 @@ def measure(duration_s=10.0, rate_hz=20.0, seed=1, epochs_cap=None):
     t0 = time.perf_counter()
     done = server.serve(reqs)
     makespan = time.perf_counter() - t0
+    server.close()
     lat = np.array([r.completed - r.arrival_abs for r in done]) * 1e3
     cs = sum(r.scenario.channel_samples for r in done)
     return {"requests": len(done), "rate_hz": rate_hz, "arrival_window_s": duration_s, "makespan_s": makespan,

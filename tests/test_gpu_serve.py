@@ -32,6 +32,8 @@ def test_requests_in_the_mix_equal_alone(monkeypatch):
         assert r.completed >= r.arrival_abs
     strong = [r for r in done if r.scenario.name == "strong_l1ca_iq4"][0]
     assert len(strong.adjustments()) >= 3
+    assert server.arena.used == sum(r.out.nbytes for r in done)  # only the kept outputs remain
+    server.close()
 
 
 def test_measure_reports_latency_percentiles():
