@@ -685,6 +685,11 @@ template <uint32_t F, class GainFn, class AfterSums>
 __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, AfterSums after_sums, uint32_t *err) {
     DevEngine E = load_engine(A.E);
     if (A.chunk_cap && A.chunk_cap < E.chunk) E.chunk = A.chunk_cap;
+    // Passes of equal size: as few passes as the pass size allows, the channels split evenly over them.
+    if (A.channels > E.chunk) {
+        const uint32_t passes = (A.channels + E.chunk - 1) / E.chunk;
+        E.chunk = (A.channels + passes - 1) / passes;
+    }
     const uint32_t tid = threadIdx.x;
     const uint32_t b = tid / LANES16, l = tid % LANES16;
     const uint32_t n0 = A.tile * TILE;
