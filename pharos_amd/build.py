@@ -17,7 +17,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("PHAROS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["synth_engine.hip"]
-CXX_SOURCES = ["signals.cpp", "scenario.cpp", "run_facade.cpp"]
+CXX_SOURCES = ["signals.cpp", "scenario.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off", "-fno-fast-math",
@@ -49,8 +49,6 @@ def build(verbose=True):
             objs.append(obj)
         for src in CXX_SOURCES:
             path = os.path.join(CSRC, src)
-            if not os.path.exists(path):
-                continue
             obj = os.path.join(tmp, src + ".o")
             _run(["g++", *CXX_FLAGS, "-c", path, "-o", obj])
             objs.append(obj)
