@@ -546,7 +546,8 @@ __device__ __forceinline__ void channel_dispatch(uint32_t axis, float (&ar)[STEP
     else channel_samples<AXIS_IM, false, WAVE>(ar, ai, pos, step, cre, cim, wre, wim, 0, 0, 0);
 }
 
-// The loop of a lead channel: rare, so one copy serves both waves.
+// The loop of a lead channel: one copy serves both waves, the slot address in a register (per-wave copies measured
+// no faster and add 7 KB of code).
 __device__ __forceinline__ void lead_dispatch(uint32_t axis, float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos,
                                               uint64_t step, float cre, float cim, float wre, float wim,
                                               uint64_t lead_add, uint32_t lead_bits, uint32_t slot) {
