@@ -5,7 +5,8 @@ The tunable knobs are those of pharos_synth_tuning:
 - channels staged per LDS pass of a tile (8-32).
 
 For a workload shape, the tuner times every candidate on a short sample of the workload itself: the first
-`sample_epochs` epochs, one batched launch, HIP events on the launch stream. It keeps the fastest candidate in a
+`sample_epochs` epochs, or as many as give MIN_TILES work items (a sample must fill the GPU several times over, or
+every shape times alike), one batched launch, HIP events on the launch stream. It keeps the fastest candidate in a
 JSON cache keyed by the shape. The shape key is (samples per epoch, format, channel count rounded up to a multiple
 of 8, job count bucket). Every candidate produces the same bytes, so tuning changes speed only.
 """
@@ -16,6 +17,8 @@ import time
 from . import _lib
 
 WORKGROUPS = (1, 2, 3, 4)
+MIN_TILES = 16384       # work items a tuning sample holds at least (16 per workgroup at 4 per CU on 256 CUs)
+TILE = 4096             # samples of a work item
 CHANNELS_PER_PASS = (8, 16, 24, 32)
 CANDIDATES = [(w, c) for w in WORKGROUPS for c in CHANNELS_PER_PASS]
 
@@ -59,6 +62,8 @@ def tune(scenarios, sample_epochs=300, repeats=2, cache_path=None, stream=None):
     from .engine import Job, Stream
     t0 = time.perf_counter()
     stream = stream or Stream()
+    tiles_per_epoch = sum(-(-s.samples_per_epoch // TILE) for s in scenarios)
+    sample_epochs = max(sample_epochs, -(-MIN_TILES // max(1, tiles_per_epoch)))
     jobs = [Job(s, epochs=min(s.epochs, sample_epochs)) for s in scenarios]
     _time(jobs, 0, 0, stream, 1)  # warm the code and the caches
     times = {f"{w}x{c}": _time(jobs, w, c, stream, repeats) for w, c in CANDIDATES}
