@@ -95,8 +95,13 @@ struct BatchState {
     uint32_t *err;
     uint32_t queue_words_offset;   // LDS byte offset of the work-queue words
     uint32_t chunk_cap;            // channels per LDS pass at most (0: each engine's own)
+    const uint32_t *seg_of_block;  // the segment holding item k * SEG_BLOCK, for k <= total items / SEG_BLOCK
 };
+constexpr uint32_t SEG_BLOCK = 256;
 
+// The segment holding `item`: the cursor's segment or the next one, else from the table of every SEG_BLOCK-th
+// item's segment and a walk forward (segments hold their items in order, so a few steps at most; a binary search
+// over the rest if the walk runs long).
 __device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
     const auto segs = gmem(S.segs);
     if (hint < S.nsegs && segs[hint].first_item <= item &&
@@ -105,7 +110,12 @@ __device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hi
     if (hint + 1 < S.nsegs && segs[hint + 1].first_item <= item &&
         (hint + 2 == S.nsegs || segs[hint + 2].first_item > item))
         return hint + 1;
-    uint32_t lo = 0, hi = S.nsegs;  // first_item[lo] <= item < first_item[hi]
+    uint32_t lo = gmem(S.seg_of_block)[item / SEG_BLOCK];
+    for (int k = 0; k < 4; k++) {
+        if (lo + 1 == S.nsegs || segs[lo + 1].first_item > item) return lo;
+        lo++;
+    }
+    uint32_t hi = S.nsegs;  // first_item[lo] <= item < first_item[hi]
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) / 2;
         if (segs[mid].first_item <= item) lo = mid; else hi = mid;
@@ -774,7 +784,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
           p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long) * ngroups),
           p_count = carve(sizeof(unsigned long long) * nsegs), p_gain = carve(sizeof(unsigned long long) * nsegs),
           p_agc = carve(sizeof(DevAgc) * njobs), p_adj = carve(sizeof(DevAdjust) * (size_t)adjust_cap * njobs),
-          p_err = carve(64);
+          p_err = carve(64), p_sob = carve(sizeof(uint32_t) * (items / SEG_BLOCK + 1));
     std::vector<uint8_t> host(total, 0);
     memcpy(host.data() + p_jobs.off, dj.data(), p_jobs.size);
     memcpy(host.data() + p_segs.off, segs.data(), p_segs.size);
@@ -786,6 +796,15 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
         memcpy(host.data() + p_gain.off + k * sizeof w, &w, sizeof w);
     }
     memcpy(host.data() + p_agc.off, agc.data(), p_agc.size);
+    {
+        uint32_t *sob = (uint32_t *)(host.data() + p_sob.off);
+        uint32_t k = 0;
+        for (uint64_t b = 0; b <= items / SEG_BLOCK; b++) {
+            const uint64_t it = b * SEG_BLOCK;
+            while (k + 1 < nsegs && segs[k + 1].first_item <= it) k++;
+            sob[b] = k;
+        }
+    }
     for (uint32_t g = 0; g < ngroups; g++) {
         const unsigned long long q0 = groups[g].begin;
         memcpy(host.data() + p_queue.off + g * sizeof q0, &q0, sizeof q0);
@@ -813,6 +832,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.adjust = (DevAdjust *)(b8 + p_adj.off);
     S.adjust_cap = adjust_cap;
     S.err = (uint32_t *)(b8 + p_err.off);
+    S.seg_of_block = (const uint32_t *)(b8 + p_sob.off);
     S.queue_words_offset = 0;
     S.chunk_cap = tuning ? std::min<uint32_t>(tuning->channels_per_pass, MAX_CHUNK) : 0;
     h->d_agc = S.agc;
