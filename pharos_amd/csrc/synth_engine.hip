@@ -603,7 +603,10 @@ struct pharos_synth_batch_handle {
     std::vector<uint32_t> job_first_seg, job_seg_count, seg_job;
     uint32_t *d_err = nullptr;
     hipEvent_t done = nullptr;
-    std::vector<uint8_t> upload;  // host image of the batch tables, alive until the copy has run
+    // Pinned host image of the batch tables: uploaded before the launch, and the result words (AGC states, segment
+    // counts, error flags) copied back into it after the launch, both on the batch's stream.
+    uint8_t *pinned = nullptr;
+    size_t pinned_bytes = 0, at_agc = 0, at_count = 0, at_err = 0;
 };
 
 // Batch-state blocks outlive their batch in a small per-process cache: a serving loop submits a batch per
@@ -613,6 +616,29 @@ struct SpareBlock { int device; void *p; size_t n; };
 static std::mutex g_spare_lock;
 static std::vector<SpareBlock> g_spare;
 constexpr size_t SPARE_MAX = 4, SPARE_MIN_BYTES = 1 << 20;
+// Pinned host images, cached the same way (a pinned allocation costs far more than a small batch's copies).
+struct SparePinned { uint8_t *p; size_t n; };
+static std::vector<SparePinned> g_spare_pinned;
+constexpr size_t SPARE_PINNED_MAX = 8, SPARE_PINNED_MIN_BYTES = 1 << 20;
+
+static uint8_t *pinned_take(size_t n, size_t *got) {
+    {
+        std::lock_guard<std::mutex> guard(g_spare_lock);
+        for (size_t i = 0; i < g_spare_pinned.size(); i++) {
+            if (g_spare_pinned[i].n >= n) {
+                uint8_t *p = g_spare_pinned[i].p;
+                *got = g_spare_pinned[i].n;
+                g_spare_pinned.erase(g_spare_pinned.begin() + (long)i);
+                return p;
+            }
+        }
+    }
+    const size_t bytes = std::max(n, SPARE_PINNED_MIN_BYTES);
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    *got = bytes;
+    return (uint8_t *)p;
+}
 
 static void batch_free(pharos_synth_batch_handle *h) {
     if (!h) return;
@@ -622,6 +648,11 @@ static void batch_free(pharos_synth_batch_handle *h) {
         std::lock_guard<std::mutex> guard(g_spare_lock);
         if (g_spare.size() < SPARE_MAX) g_spare.push_back({h->device, h->owned[i], h->owned_bytes[i]});
         else (void)hipFree(h->owned[i]);
+    }
+    if (h->pinned) {
+        std::lock_guard<std::mutex> guard(g_spare_lock);
+        if (g_spare_pinned.size() < SPARE_PINNED_MAX) g_spare_pinned.push_back({h->pinned, h->pinned_bytes});
+        else (void)hipHostFree(h->pinned);
     }
     delete h;
 }
@@ -776,28 +807,38 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
         seg_gain[f] = J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE ? (float)J.agc->gain : J.constant_gain;
         seg_ready[f] = 1;
     }
-    // Device buffers: one allocation carved into the pieces.
+    // Device buffers: one allocation carved into the pieces.  The tables the kernel reads come first, then the
+    // result words (copied back after the launch), then the adjustment logs (written by the device only, read back
+    // as far as they were filled).
     struct Piece { size_t off, size; };
     size_t total = 0;
     auto carve = [&](size_t n) { size_t o = (total + 255) & ~(size_t)255; total = o + n; return Piece{o, n}; };
     Piece p_jobs = carve(sizeof(DevJob) * njobs), p_segs = carve(sizeof(DevSegment) * nsegs),
           p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long) * ngroups),
-          p_count = carve(sizeof(unsigned long long) * nsegs), p_gain = carve(sizeof(unsigned long long) * nsegs),
-          p_agc = carve(sizeof(DevAgc) * njobs), p_adj = carve(sizeof(DevAdjust) * (size_t)adjust_cap * njobs),
-          p_err = carve(64), p_sob = carve(sizeof(uint32_t) * (items / SEG_BLOCK + 1));
-    std::vector<uint8_t> host(total, 0);
-    memcpy(host.data() + p_jobs.off, dj.data(), p_jobs.size);
-    memcpy(host.data() + p_segs.off, segs.data(), p_segs.size);
-    memcpy(host.data() + p_next.off, seg_next.data(), p_next.size);
+          p_gain = carve(sizeof(unsigned long long) * nsegs), p_sob = carve(sizeof(uint32_t) * (items / SEG_BLOCK + 1)),
+          p_agc = carve(sizeof(DevAgc) * njobs), p_count = carve(sizeof(unsigned long long) * nsegs),
+          p_err = carve(64);
+    const size_t image = total;
+    Piece p_adj = carve(sizeof(DevAdjust) * (size_t)adjust_cap * njobs);
+    h->pinned = pinned_take(image, &h->pinned_bytes);
+    if (!h->pinned) { set_error("hipHostMalloc(%zu) failed", image); batch_free(h); return ST_NOMEM; }
+    h->at_agc = p_agc.off;
+    h->at_count = p_count.off;
+    h->at_err = p_err.off;
+    uint8_t *host = h->pinned;
+    memset(host, 0, image);
+    memcpy(host + p_jobs.off, dj.data(), p_jobs.size);
+    memcpy(host + p_segs.off, segs.data(), p_segs.size);
+    memcpy(host + p_next.off, seg_next.data(), p_next.size);
     for (uint32_t k = 0; k < nsegs; k++) {
         uint32_t bits;
         memcpy(&bits, &seg_gain[k], sizeof bits);
         const unsigned long long w = (seg_ready[k] ? READY_BIT : 0ull) | bits;
-        memcpy(host.data() + p_gain.off + k * sizeof w, &w, sizeof w);
+        memcpy(host + p_gain.off + k * sizeof w, &w, sizeof w);
     }
-    memcpy(host.data() + p_agc.off, agc.data(), p_agc.size);
+    memcpy(host + p_agc.off, agc.data(), p_agc.size);
     {
-        uint32_t *sob = (uint32_t *)(host.data() + p_sob.off);
+        uint32_t *sob = (uint32_t *)(host + p_sob.off);
         uint32_t k = 0;
         for (uint64_t b = 0; b <= items / SEG_BLOCK; b++) {
             const uint64_t it = b * SEG_BLOCK;
@@ -807,14 +848,13 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     }
     for (uint32_t g = 0; g < ngroups; g++) {
         const unsigned long long q0 = groups[g].begin;
-        memcpy(host.data() + p_queue.off + g * sizeof q0, &q0, sizeof q0);
+        memcpy(host + p_queue.off + g * sizeof q0, &q0, sizeof q0);
     }
     void *base;
     int st = batch_alloc(h, &base, total);
     if (st) { batch_free(h); return st; }
     uint8_t *b8 = (uint8_t *)base;
-    h->upload = std::move(host);
-    if (hipMemcpyAsync(base, h->upload.data(), total, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+    if (hipMemcpyAsync(base, host, image, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
         set_error("batch upload failed");
         batch_free(h);
         return ST_INTERNAL;
@@ -872,6 +912,13 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
             return ST_INTERNAL;
         }
     }
+    const size_t back = h->at_err + 64 - h->at_agc;
+    if (hipMemcpyAsync(h->pinned + h->at_agc, b8 + h->at_agc, back, hipMemcpyDeviceToHost, h->stream) != hipSuccess) {
+        set_error("batch read-back failed");
+        (void)hipStreamSynchronize(h->stream);
+        batch_free(h);
+        return ST_INTERNAL;
+    }
     if (hipEventCreateWithFlags(&h->done, hipEventDisableTiming) != hipSuccess || hipEventRecord(h->done, h->stream) != hipSuccess) {
         set_error("event record failed");
         (void)hipStreamSynchronize(h->stream);
@@ -895,20 +942,18 @@ extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
     if (!h) { set_error("null handle"); return ST_ARG; }
     (void)hipSetDevice(h->device);
     const uint32_t njobs = (uint32_t)h->jobs.size();
-    std::vector<DevAgc> agc(njobs);
-    std::vector<DevAdjust> adj((size_t)h->adjust_cap * njobs);
-    std::vector<unsigned long long> seg_count(h->nsegs);
-    uint32_t err = 0;
     int status = ST_OK;
-    if (hipEventSynchronize(h->done) != hipSuccess ||
-        hipMemcpy(agc.data(), h->d_agc, sizeof(DevAgc) * njobs, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(adj.data(), h->d_adjust, sizeof(DevAdjust) * adj.size(), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(seg_count.data(), h->d_seg_count, sizeof(unsigned long long) * h->nsegs, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&err, h->d_err, sizeof err, hipMemcpyDeviceToHost) != hipSuccess) {
-        set_error("batch failed: %s", hipGetErrorString(hipGetLastError()));
+    const hipError_t done = hipEventSynchronize(h->done);
+    if (done != hipSuccess) {
+        set_error("batch failed: %s", hipGetErrorString(done));
         batch_free(h);
         return ST_INTERNAL;
     }
+    const DevAgc *agc = (const DevAgc *)(h->pinned + h->at_agc);
+    const unsigned long long *seg_count = (const unsigned long long *)(h->pinned + h->at_count);
+    uint32_t err;
+    memcpy(&err, h->pinned + h->at_err, sizeof err);
+    std::vector<DevAdjust> adj;
     std::vector<uint64_t> job_clip(njobs, 0);
     for (uint32_t s = 0; s < h->nsegs; s++) job_clip[h->seg_job[s]] += seg_count[s] & CLIP_MASK;
     for (uint32_t j = 0; j < njobs; j++) {
@@ -920,10 +965,17 @@ extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
             J.agc->clipped = agc[j].clipped;
             J.agc->components = agc[j].components;
             J.agc->adjustments = prior + agc[j].adjustments;
-            if (J.adjustments) {
-                const uint32_t n = std::min(agc[j].adjustments, std::min(J.adjustments_cap, h->adjust_cap));
+            const uint32_t n = std::min(agc[j].adjustments, std::min(J.adjustments_cap, h->adjust_cap));
+            if (J.adjustments && n) {
+                adj.resize(n);
+                if (hipMemcpy(adj.data(), h->d_adjust + (size_t)j * h->adjust_cap, sizeof(DevAdjust) * n,
+                              hipMemcpyDeviceToHost) != hipSuccess) {
+                    set_error("adjustment read-back failed: %s", hipGetErrorString(hipGetLastError()));
+                    status = ST_INTERNAL;
+                    break;
+                }
                 for (uint32_t k = 0; k < n; k++) {
-                    const DevAdjust &a = adj[(size_t)j * h->adjust_cap + k];
+                    const DevAdjust &a = adj[k];
                     J.adjustments[k].epoch = a.epoch;
                     J.adjustments[k].gain = a.gain;
                     J.adjustments[k].clip_rate = a.rate;
