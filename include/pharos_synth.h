@@ -99,6 +99,9 @@ typedef struct pharos_synth_adjustment {
 
 uint32_t pharos_synth_abi_version(void);
 const char *pharos_synth_last_error(void);
+/* The f64 turn table every engine uploads (rotor.zig:64-72), as (cos, sin) pairs for entries 0..count-1 (count at
+ * most 4096), built on the host with no device call, so it can be checked anywhere. */
+int pharos_synth_turn_table(double *out, uint32_t count);
 
 /* Creates an engine on HIP device `device` (the tables are copied to it).  Replaces the table part of
  * Channels.create (src/products/run/engine/channels.zig:127-203) on the device side. */

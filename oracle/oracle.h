@@ -6,9 +6,10 @@
  * Only tests, the smoke check and the benchmark's CPU-baseline leg may load it; no product path does.
  *
  * Every function cites the reference file:line it follows.  Pins (known-answer and property tests) live
- * in tests/test_oracle_*.py.  Parity of three functions is unpinned at the last bit against the reference
- * itself because they go through Zig's own libm (turn-table cos/sin, derive's pow, Db.linear's pow): the
- * oracle uses the host libm, and tests bound those differences instead.
+ * in tests/test_oracle_*.py.  Three values come from Zig's math library in the reference:
+ * - the turn table's cos/sin: restated here with the fdlibm algorithm that Zig's comptime @cos/@sin run;
+ * - derive's pow and Db.linear's pow: the host pow, whose f32 roundings tests show equal to those of Zig's pow
+ *   algorithm over dense input grids (tests/test_oracle_libm.py).
  */
 #ifndef PHAROS_ORACLE_H
 #define PHAROS_ORACLE_H

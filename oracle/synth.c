@@ -44,11 +44,69 @@ uint32_t orc_sub_tile_for(uint32_t spe) { return align16(spe < SUB_TILE ? spe : 
 static double TURN_C[4096], TURN_S[4096];
 static int turn_ready;
 static pthread_once_t turn_once = PTHREAD_ONCE_INIT;
-static void build_turn(void) { /* rotor.zig:64-72 (host libm in place of Zig's comptime cos/sin) */
+/* Zig's comptime @cos / @sin (rotor.zig:69-70) run the fdlibm algorithm (musl, Zig's compiler_rt): Sun's
+ * __rem_pio2 with the kernels on [-pi/4, pi/4].  It is not correctly rounded, so the host libm would give another
+ * table; this is that algorithm for 0 <= x < 2pi. */
+static uint32_t fd_hi(double x) { uint64_t b; memcpy(&b, &x, 8); return (uint32_t)(b >> 32 & 0x7fffffff); }
+static int fd_exp(double x) { uint64_t b; memcpy(&b, &x, 8); return (int)(b >> 52 & 0x7ff); }
+static double fd_ksin(double x, double y, int iy) {
+    double z = x * x, w = z * z, v = z * x;
+    double r = 8.33333333332248946124e-03 + z * (-1.98412698298579493134e-04 + z * 2.75573137070700676789e-06) +
+               z * w * (-2.50507602534068634195e-08 + z * 1.58969099521155010221e-10);
+    if (!iy) return x + v * (-1.66666666666666324348e-01 + z * r);
+    return x - ((z * (0.5 * y - v * r) - y) - v * -1.66666666666666324348e-01);
+}
+static double fd_kcos(double x, double y) {
+    double z = x * x, w = z * z;
+    double r = z * (4.16666666666666019037e-02 + z * (-1.38888888888741095749e-03 + z * 2.48015872894767294178e-05)) +
+               w * w * (-2.75573143513906633035e-07 + z * (2.08757232129817482790e-09 + z * -1.13596475577881948265e-11));
+    double hz = 0.5 * z, a = 1.0 - hz;
+    return a + (((1.0 - a) - hz) + (z * r - x * y));
+}
+static void fd_sincos(double x, double *s, double *c) {
+    const double P1 = 1.57079632673412561417e+00, P1T = 6.07710050650619224932e-11, P2 = 6.07710050630396597660e-11,
+                 P2T = 2.02226624879595063154e-21, P3 = 2.02226624871116645580e-21, P3T = 8.47842766036889956997e-32;
+    uint32_t ix = fd_hi(x);
+    if (ix <= 0x3fe921fb) {
+        *s = ix < 0x3e500000 ? x : fd_ksin(x, 0, 0);
+        *c = ix < 0x3e46a09e ? 1.0 : fd_kcos(x, 0);
+        return;
+    }
+    int n = -1;
+    double y0 = 0, y1 = 0;
+    if (ix <= 0x400f6a7a) { if ((ix & 0xfffff) != 0x921fb) n = ix <= 0x4002d97c ? 1 : 2; }
+    else if (ix <= 0x401c463b) { if (ix <= 0x4015fdbc) { if (ix != 0x4012d97c) n = 3; } else if (ix != 0x401921fb) n = 4; }
+    if (n > 0) {
+        double z = x - n * P1;
+        y0 = z - n * P1T;
+        y1 = (z - y0) - n * P1T;
+    } else {
+        double fn = x * 6.36619772367581382433e-01 + 6755399441055744.0 - 6755399441055744.0;
+        n = (int)fn;
+        double r = x - fn * P1, w = fn * P1T;
+        if (r - w < -7.85398163397448278999e-01) { n--; fn--; r = x - fn * P1; w = fn * P1T; }
+        else if (r - w > 7.85398163397448278999e-01) { n++; fn++; r = x - fn * P1; w = fn * P1T; }
+        y0 = r - w;
+        int ex = (int)(ix >> 20);
+        if (ex - fd_exp(y0) > 16) {
+            double t = r;
+            w = fn * P2; r = t - w; w = fn * P2T - ((t - r) - w); y0 = r - w;
+            if (ex - fd_exp(y0) > 49) { t = r; w = fn * P3; r = t - w; w = fn * P3T - ((t - r) - w); y0 = r - w; }
+        }
+        y1 = (r - y0) - w;
+    }
+    double ks = fd_ksin(y0, y1, 1), kc = fd_kcos(y0, y1);
+    switch (n & 3) {
+    case 0: *s = ks; *c = kc; break;
+    case 1: *s = kc; *c = -ks; break;
+    case 2: *s = -ks; *c = -kc; break;
+    default: *s = -kc; *c = ks; break;
+    }
+}
+static void build_turn(void) { /* rotor.zig:64-72 */
     for (int i = 0; i < 4096; i++) {
         double a = 2.0 * 3.14159265358979323846 * (double)i / 4096.0;
-        TURN_C[i] = cos(a);
-        TURN_S[i] = sin(a);
+        fd_sincos(a, &TURN_S[i], &TURN_C[i]);
     }
     turn_ready = 1;
 }

@@ -97,6 +97,7 @@ def lib():
     sig = {
         "pharos_synth_abi_version": (U32, []),
         "pharos_synth_last_error": (C.c_char_p, []),
+        "pharos_synth_turn_table": (C.c_int, [P(C.c_double), U32]),
         "pharos_synth_create": (C.c_int, [P(SynthConfig), P(SynthTable), U32, C.c_int, P(VP)]),
         "pharos_synth_destroy": (None, [VP]),
         "pharos_synth_bytes_per_epoch": (U64, [VP]),

@@ -326,14 +326,115 @@ static void engine_free(pharos_synth_engine *e) {
     delete e;
 }
 
-// rotor.zig:64-72: the f64 turn table, from the host libm.
+// rotor.zig:64-72: the f64 turn table.  Zig evaluates its comptime @cos / @sin with the fdlibm algorithm (Sun
+// Microsystems, 1993; musl's and Zig's compiler_rt sin and cos), which is not correctly rounded: on about 3 % of
+// the table's angles it returns the other neighbour of the exact value.  So the table is built with that algorithm
+// here rather than with the host libm: the argument split __rem_pio2 and the kernels __sin / __cos on [-pi/4, pi/4].
+namespace fdlibm {
+static inline uint32_t high_word(double x) { uint64_t b; memcpy(&b, &x, 8); return (uint32_t)(b >> 32); }
+static inline uint32_t exponent(double x) { uint64_t b; memcpy(&b, &x, 8); return (uint32_t)(b >> 52) & 0x7ff; }
+
+static double k_sin(double x, double y, int iy) {
+    const double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03,
+                 S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06,
+                 S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+    const double z = x * x, w = z * z;
+    const double r = S2 + z * (S3 + z * S4) + z * w * (S5 + z * S6);
+    const double v = z * x;
+    if (iy == 0) return x + v * (S1 + z * r);
+    return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+}
+
+static double k_cos(double x, double y) {
+    const double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03,
+                 C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07,
+                 C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
+    const double z = x * x, w = z * z;
+    const double r = z * (C1 + z * (C2 + z * C3)) + w * w * (C4 + z * (C5 + z * C6));
+    const double hz = 0.5 * z, a = 1.0 - hz;
+    return a + (((1.0 - a) - hz) + (z * r - x * y));
+}
+
+// x = n * pi/2 + y[0] + y[1] for 0 < x < 2^20 * pi/2 (positive arguments only: the table's).
+static int rem_pio2(double x, double *y) {
+    const double pio2_1 = 1.57079632673412561417e+00, pio2_1t = 6.07710050650619224932e-11,
+                 pio2_2 = 6.07710050630396597660e-11, pio2_2t = 2.02226624879595063154e-21,
+                 pio2_3 = 2.02226624871116645580e-21, pio2_3t = 8.47842766036889956997e-32,
+                 invpio2 = 6.36619772367581382433e-01, pio4 = 7.85398163397448278999e-01,
+                 toint = 6755399441055744.0;  // 1.5 * 2^52
+    const uint32_t ix = high_word(x) & 0x7fffffff;
+    auto near = [&](int k) {
+        const double z = x - k * pio2_1;
+        y[0] = z - k * pio2_1t;
+        y[1] = (z - y[0]) - k * pio2_1t;
+        return k;
+    };
+    if (ix <= 0x400f6a7a) {                                         // x ~<= 5pi/4
+        if ((ix & 0xfffff) != 0x921fb) return near(ix <= 0x4002d97c ? 1 : 2);
+    } else if (ix <= 0x401c463b) {                                  // x ~<= 9pi/4
+        if (ix <= 0x4015fdbc) { if (ix != 0x4012d97c) return near(3); }
+        else if (ix != 0x401921fb) return near(4);
+    }
+    double fn = x * invpio2 + toint - toint;
+    int n = (int)fn;
+    double r = x - fn * pio2_1, w = fn * pio2_1t;
+    if (r - w < -pio4) { n--; fn--; r = x - fn * pio2_1; w = fn * pio2_1t; }
+    else if (r - w > pio4) { n++; fn++; r = x - fn * pio2_1; w = fn * pio2_1t; }
+    y[0] = r - w;
+    const int ex = (int)(ix >> 20);
+    if (ex - (int)exponent(y[0]) > 16) {
+        double t = r;
+        w = fn * pio2_2;
+        r = t - w;
+        w = fn * pio2_2t - ((t - r) - w);
+        y[0] = r - w;
+        if (ex - (int)exponent(y[0]) > 49) {
+            t = r;
+            w = fn * pio2_3;
+            r = t - w;
+            w = fn * pio2_3t - ((t - r) - w);
+            y[0] = r - w;
+        }
+    }
+    y[1] = (r - y[0]) - w;
+    return n;
+}
+
+// sin and cos of 0 <= x < 2^20 * pi/2.
+static void sincos(double x, double *s, double *c) {
+    const uint32_t ix = high_word(x) & 0x7fffffff;
+    if (ix <= 0x3fe921fb) {
+        *s = ix < 0x3e500000 ? x : k_sin(x, 0.0, 0);
+        *c = ix < 0x3e46a09e ? 1.0 : k_cos(x, 0.0);
+        return;
+    }
+    double y[2];
+    const int n = rem_pio2(x, y);
+    const double ks = k_sin(y[0], y[1], 1), kc = k_cos(y[0], y[1]);
+    switch (n & 3) {
+    case 0: *s = ks; *c = kc; break;
+    case 1: *s = kc; *c = -ks; break;
+    case 2: *s = -ks; *c = -kc; break;
+    default: *s = -kc; *c = ks; break;
+    }
+}
+}  // namespace fdlibm
+
 static void build_turn(std::vector<double> &t) {
     t.resize(2 * 4096);
     for (int i = 0; i < 4096; i++) {
-        double a = 2.0 * 3.14159265358979323846 * (double)i / 4096.0;
-        t[2 * i] = cos(a);
-        t[2 * i + 1] = sin(a);
+        const double a = 2.0 * 3.14159265358979323846 * (double)i / 4096.0;
+        fdlibm::sincos(a, &t[2 * i + 1], &t[2 * i]);
     }
+}
+
+extern "C" int pharos_synth_turn_table(double *out, uint32_t count) {
+    g_error[0] = 0;
+    if (!out || count > 4096) { set_error("turn table: null output or more than 4096 entries"); return ST_ARG; }
+    std::vector<double> t;
+    build_turn(t);
+    memcpy(out, t.data(), sizeof(double) * 2 * count);
+    return ST_OK;
 }
 
 // The turn table is the same for every engine: one copy per device, made on first use and kept for the life of the
