@@ -36,6 +36,7 @@ def _assert_outputs_equal(a, b, chunk=1 << 28):
 
 @pytest.mark.parametrize("name,make,sampled", [
     ("c1", S.c1, (0, 1, 777, 29_999, 30_000, 59_999)),
+    ("c2", S.c2, (0, 12_345, 19_999)),
     ("c0", S.c0, (0, 4_321, 9_999)),
 ])
 def test_full_run(name, make, sampled):
