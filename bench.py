@@ -12,7 +12,7 @@ adaptive AGC -- rows and masks resident in HBM before the timed region, the 3.69
   serving      p50/p99 latency of a Poisson request stream (--serve)
 
 --workload c3 instead measures the uneven batch: 256 seeded scenarios (4-48 channels, 4.092-61.44 Msps,
-IQ2-IQ16, durations cut to --c3-epochs-cap epochs) as one batched launch per format, against per-scenario plain
+IQ2-IQ16, durations cut to --c3-epochs-cap epochs) as one batched persistent launch, against per-scenario plain
 launches and per-scenario graphs.
 """
 import argparse
@@ -255,7 +255,7 @@ def bench_c3(args):
         "dtype": "f32", "data": "synthetic (256 seeded scenarios; rows, masks and tables resident in HBM)",
         "config": {"workload": "uneven_256_scenarios", "scenarios": len(scs), "epochs_cap": cap,
                    "channel_samples_per_step": cs, "output_bytes": sum(s.epochs * s.bytes_per_epoch for s in scs),
-                   "formats": fmts, "form": "one batched persistent launch per output format"},
+                   "formats": fmts, "form": "one batched persistent launch (each job's quantiser picked per tile)"},
         "roofline": {"bound": "valu", "achieved": flops / kern / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": flops / kern / 1e12 / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "synth_batch_kernel",
                      "kernel_ms": kern * 1e3},

@@ -14,6 +14,7 @@ constexpr uint32_t STEPS = RESEED / LANES16;  // 32 advances per block
 constexpr uint32_t TILE_BLOCKS = 8;
 constexpr uint32_t THREADS = TILE_BLOCKS * LANES16;  // 128 = two wave64
 constexpr uint32_t TILE = TILE_BLOCKS * RESEED;      // 4096 samples
+constexpr uint32_t FORMAT_ANY = 0xff;  // a batch kernel instance whose jobs may have any output format
 constexpr uint32_t MAX_CHUNK = 32;   // channels staged per LDS pass (at most THREADS / 4: one uint4 of a row per thread)
 constexpr uint32_t GROUP = THREADS / MAX_CHUNK;  // threads that set up and stage one channel of a pass
 constexpr uint32_t LDS_LIMIT = 40960;  // LDS per workgroup that keeps four workgroups on a CU's 160 KiB

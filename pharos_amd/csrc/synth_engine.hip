@@ -811,10 +811,8 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     h->stream = (hipStream_t)stream_;
     h->jobs.assign(jobs, jobs + njobs);
 
-    // Windows end where the AGC evaluates (multiples of 100 epochs, gain.zig:5,90-93).  Jobs run in groups of one
-    // output format, one persistent launch per group (a launch carries only its format's quantiser).  Within a
-    // group items run round by round: every job's first window, then every job's second, so a window's gain is
-    // ready long before it is needed.
+    // Windows end where the AGC evaluates (multiples of 100 epochs, gain.zig:5,90-93).  Items run round by round:
+    // every job's first window, then every job's second, so a window's gain is ready long before it is needed.
     struct Win { uint32_t job, off, epochs; };
     std::vector<std::vector<Win>> per_job(njobs);
     for (uint32_t j = 0; j < njobs; j++) {
@@ -828,14 +826,13 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
             off += n;
         }
     }
+    // One launch: a single-format batch runs the kernel instance with only that format's quantiser, a mixed one
+    // the instance that picks each job's quantiser per tile.
     struct Group { uint32_t format, lds; uint64_t begin, end; };
-    std::vector<Group> groups;
-    for (uint32_t j = 0; j < njobs; j++) {
-        const uint32_t f = jobs[j].engine->view.format;
-        bool seen = false;
-        for (const Group &g : groups) seen |= g.format == f;
-        if (!seen) groups.push_back({f, 0, 0, 0});
-    }
+    uint32_t batch_format = jobs[0].engine->view.format;
+    for (uint32_t j = 1; j < njobs; j++)
+        if (jobs[j].engine->view.format != batch_format) batch_format = FORMAT_ANY;
+    std::vector<Group> groups{{batch_format, 0, 0, 0}};
     std::vector<DevSegment> segs;
     std::vector<uint32_t> seg_next;
     std::vector<uint32_t> last_seg(njobs, 0xffffffffu);
@@ -846,13 +843,12 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
       g.begin = items;
       size_t rounds = 0;
       for (uint32_t j = 0; j < njobs; j++) {
-          if (jobs[j].engine->view.format != g.format) continue;
           rounds = std::max(rounds, per_job[j].size());
           g.lds = std::max(g.lds, jobs[j].engine->view.lds_bytes);
       }
       for (size_t r = 0; r < rounds; r++) {
         for (uint32_t j = 0; j < njobs; j++) {
-            if (jobs[j].engine->view.format != g.format || r >= per_job[j].size()) continue;
+            if (r >= per_job[j].size()) continue;
             const Win &w = per_job[j][r];
             DevSegment s;
             s.first_item = items;
@@ -994,7 +990,8 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
         case PHAROS_SYNTH_IQ4: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ4>; break;
         case PHAROS_SYNTH_IQ8: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ8>; break;
         case PHAROS_SYNTH_IQ16: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ16>; break;
-        default: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_F32>; break;
+        case PHAROS_SYNTH_F32: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_F32>; break;
+        default: fn = (const void *)synth_batch_kernel<FORMAT_ANY>; break;
         }
         uint32_t per_cu = tuning && tuning->workgroups_per_cu ? tuning->workgroups_per_cu : 0;
         if (!per_cu) {

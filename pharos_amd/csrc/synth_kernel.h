@@ -679,7 +679,7 @@ struct TileArgs {
 };
 
 
-// One work item of an engine whose output format is F.  AfterSums is called by every thread when the last chunk's
+// One work item of an engine whose output format is F (FORMAT_ANY: the engine's own, chosen per tile).  AfterSums is called by every thread when the last chunk's
 // sums are done (the persistent kernel fetches its next item there); GainFn by every thread before quantising (the
 // persistent kernel waits there for the AGC of the previous window).  Returns the clipped components of the tile.
 template <uint32_t F, class GainFn, class AfterSums>
@@ -794,7 +794,18 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
     const uint32_t valid = remain >= TILE ? TILE : remain;
     const uint64_t abs0 = A.epoch * (uint64_t)E.spe + n0;
     uint8_t *dst = A.out_epoch + bytes_for(E.format, n0);
-    uint32_t clip = epilogue<F>(L, E, valid, abs0, gain, dst);
+    uint32_t clip;
+    if constexpr (F == FORMAT_ANY) {
+        switch (E.format) {
+        case PHAROS_SYNTH_IQ2: clip = epilogue<PHAROS_SYNTH_IQ2>(L, E, valid, abs0, gain, dst); break;
+        case PHAROS_SYNTH_IQ4: clip = epilogue<PHAROS_SYNTH_IQ4>(L, E, valid, abs0, gain, dst); break;
+        case PHAROS_SYNTH_IQ8: clip = epilogue<PHAROS_SYNTH_IQ8>(L, E, valid, abs0, gain, dst); break;
+        case PHAROS_SYNTH_IQ16: clip = epilogue<PHAROS_SYNTH_IQ16>(L, E, valid, abs0, gain, dst); break;
+        default: clip = epilogue<PHAROS_SYNTH_F32>(L, E, valid, abs0, gain, dst); break;
+        }
+    } else {
+        clip = epilogue<F>(L, E, valid, abs0, gain, dst);
+    }
 
     // Clip count of the tile.
 #pragma unroll

@@ -61,3 +61,22 @@ def test_batch_from_a_mid_run_epoch():
     (job,) = run_batch([Job(sc, first_epoch=137, epochs=283, rows=rows, masks=masks)])
     assert np.array_equal(job.result(), pb)
     assert job.adjustments() == padj and job.agc.gain == pgain
+
+
+def test_every_format_in_one_launch_each_as_alone():
+    """A batch mixing all four quantised formats runs as one persistent launch that picks each job's quantiser per
+    tile; every job must still equal its plain run."""
+    jobs_sc = []
+    for k, fmt in enumerate(["IQ2", "IQ4", "IQ8", "IQ16"]):
+        sc = _strong_iq4(20 + k, spe=16_368, epochs=260 + 30 * k)
+        sc.format, sc.name = fmt, f"strong_l1ca_{fmt.lower()}"
+        jobs_sc.append(sc)
+    jobs_sc.append(S.c0(seed=12, epochs=150))
+    print("case every format", [(s.format, s.epochs) for s in jobs_sc], flush=True)
+    jobs = run_batch([Job(s) for s in jobs_sc])
+    for s, j in zip(jobs_sc, jobs):
+        pb, pc, pgain, padj = run_plain(s)
+        assert np.array_equal(j.result(), pb), s.name
+        assert j.clipped.value == int(pc.sum()), s.name
+        if s.gain_db is None:
+            assert j.adjustments() == padj and j.agc.gain == pgain, s.name
