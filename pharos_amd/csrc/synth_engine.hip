@@ -81,7 +81,7 @@ struct BatchState {
     const DevSegment *segs;
     const uint32_t *seg_next;      // the same job's next segment, or ~0
     uint32_t nsegs;
-    uint64_t total_items;          // end of this launch's items (they start where its queue starts)
+    uint64_t total_items;          // the launch's work items are 0 .. total_items - 1
     unsigned long long *queue;     // next work item
     // Per segment, two 64-bit words, each read and written only as a whole with relaxed device-scope atomics,
     // so the per-item path needs no fence:
@@ -828,25 +828,20 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     }
     // One launch: a single-format batch runs the kernel instance with only that format's quantiser, a mixed one
     // the instance that picks each job's quantiser per tile.
-    struct Group { uint32_t format, lds; uint64_t begin, end; };
-    uint32_t batch_format = jobs[0].engine->view.format;
-    for (uint32_t j = 1; j < njobs; j++)
+    uint32_t batch_format = jobs[0].engine->view.format, batch_lds = 0;
+    size_t rounds = 0;
+    for (uint32_t j = 0; j < njobs; j++) {
         if (jobs[j].engine->view.format != batch_format) batch_format = FORMAT_ANY;
-    std::vector<Group> groups{{batch_format, 0, 0, 0}};
+        batch_lds = std::max(batch_lds, jobs[j].engine->view.lds_bytes);
+        rounds = std::max(rounds, per_job[j].size());
+    }
     std::vector<DevSegment> segs;
     std::vector<uint32_t> seg_next;
     std::vector<uint32_t> last_seg(njobs, 0xffffffffu);
     h->job_first_seg.assign(njobs, 0xffffffffu);
     h->job_seg_count.assign(njobs, 0);
     uint64_t items = 0;
-    for (Group &g : groups) {
-      g.begin = items;
-      size_t rounds = 0;
-      for (uint32_t j = 0; j < njobs; j++) {
-          rounds = std::max(rounds, per_job[j].size());
-          g.lds = std::max(g.lds, jobs[j].engine->view.lds_bytes);
-      }
-      for (size_t r = 0; r < rounds; r++) {
+    for (size_t r = 0; r < rounds; r++) {
         for (uint32_t j = 0; j < njobs; j++) {
             if (r >= per_job[j].size()) continue;
             const Win &w = per_job[j][r];
@@ -866,10 +861,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
             last_seg[j] = idx;
             h->job_seg_count[j]++;
         }
-      }
-      g.end = items;
     }
-    const uint32_t ngroups = (uint32_t)groups.size();
     const uint32_t nsegs = (uint32_t)segs.size();
     h->nsegs = nsegs;
     uint32_t adjust_cap = 1;
@@ -911,7 +903,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     size_t total = 0;
     auto carve = [&](size_t n) { size_t o = (total + 255) & ~(size_t)255; total = o + n; return Piece{o, n}; };
     Piece p_jobs = carve(sizeof(DevJob) * njobs), p_segs = carve(sizeof(DevSegment) * nsegs),
-          p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long) * ngroups),
+          p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long)),
           p_gain = carve(sizeof(unsigned long long) * nsegs), p_sob = carve(sizeof(uint32_t) * (items / SEG_BLOCK + 1)),
           p_agc = carve(sizeof(DevAgc) * njobs), p_count = carve(sizeof(unsigned long long) * nsegs),
           p_err = carve(64);
@@ -943,10 +935,6 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
             sob[b] = k;
         }
     }
-    for (uint32_t g = 0; g < ngroups; g++) {
-        const unsigned long long q0 = groups[g].begin;
-        memcpy(host + p_queue.off + g * sizeof q0, &q0, sizeof q0);
-    }
     void *base;
     int st = batch_alloc(h, &base, total);
     if (st) { batch_free(h); return st; }
@@ -962,7 +950,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.seg_next = (const uint32_t *)(b8 + p_next.off);
     S.nsegs = nsegs;
     S.total_items = items;
-    S.queue = nullptr;
+    S.queue = (unsigned long long *)(b8 + p_queue.off);  // the next item; starts at 0
     S.seg_count = (unsigned long long *)(b8 + p_count.off);
     S.seg_gain = (unsigned long long *)(b8 + p_gain.off);
     S.agc = (DevAgc *)(b8 + p_agc.off);
@@ -970,22 +958,17 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.adjust_cap = adjust_cap;
     S.err = (uint32_t *)(b8 + p_err.off);
     S.seg_of_block = (const uint32_t *)(b8 + p_sob.off);
-    S.queue_words_offset = 0;
+    S.queue_words_offset = (batch_lds + 15u) & ~15u;
     S.chunk_cap = tuning ? std::min<uint32_t>(tuning->channels_per_pass, MAX_CHUNK) : 0;
     h->d_agc = S.agc;
     h->d_adjust = S.adjust;
     h->d_seg_count = S.seg_count;
     h->d_err = S.err;
 
-    for (uint32_t g = 0; g < ngroups; g++) {
-        const Group &G = groups[g];
-        BatchState Sg = S;
-        Sg.queue = (unsigned long long *)(b8 + p_queue.off) + g;
-        Sg.total_items = G.end;
-        Sg.queue_words_offset = (G.lds + 15u) & ~15u;
-        const uint32_t lds = Sg.queue_words_offset + 16;
+    {
+        const uint32_t lds = S.queue_words_offset + 16;
         const void *fn = nullptr;
-        switch (G.format) {
+        switch (batch_format) {
         case PHAROS_SYNTH_IQ2: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ2>; break;
         case PHAROS_SYNTH_IQ4: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ4>; break;
         case PHAROS_SYNTH_IQ8: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ8>; break;
@@ -1000,9 +983,9 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
             per_cu = (uint32_t)occ;
         }
         uint64_t grid = (uint64_t)cu_count(device) * per_cu;
-        if (grid > G.end - G.begin) grid = G.end - G.begin;
+        if (grid > items) grid = items;
         if (grid == 0) grid = 1;
-        void *args[] = {&Sg};
+        void *args[] = {&S};
         if (hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(THREADS), args, lds, h->stream) != hipSuccess) {
             set_error("batch launch failed");
             (void)hipStreamSynchronize(h->stream);
