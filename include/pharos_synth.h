@@ -174,6 +174,14 @@ int pharos_synth_graph_finish(pharos_synth_graph *graph, void *stream);
 uint32_t pharos_synth_graph_windows(const pharos_synth_graph *graph);
 void pharos_synth_graph_destroy(pharos_synth_graph *graph);
 
+/* The job's run written to the file at `path`, as pharos-run's file sink does (output/sink.zig:18-47, behind
+ * pharos_run_to_file): blocks of up to 100 epochs, the first ending on the 100-epoch grid, each synthesised on the
+ * device while the previous one is copied to pinned host memory and written by a writer thread.  The AGC runs
+ * between blocks as Run.pull does, and the job's agc, adjustments and clipped receive the results as after
+ * pharos_synth_batch_wait.  job->out is not used (the sink holds two blocks of device and pinned memory).
+ * *bytes_written is the number of bytes on file; a write error gives status 4 and its message. */
+int pharos_synth_to_file(const pharos_synth_job *job, const char *path, uint64_t *bytes_written);
+
 /* Device plumbing for callers without a HIP runtime of their own (the library's runtime owns the device).
  * Copies are synchronous; kind 0 host to device, 1 device to host, 2 device to device. */
 int pharos_synth_device_count(void);

@@ -10,7 +10,9 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <errno.h>
 #include <algorithm>
+#include <future>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -125,7 +127,7 @@ __device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hi
 
 // Agc.account over one window (gain.zig:19-35, called per window as engine/run.zig:240-246 does): the counts
 // accumulate, and at an evaluation epoch (a multiple of 100) the clip rate moves the gain.
-__device__ void agc_step(DevAgc &G, unsigned long long clipped, unsigned long long components,
+__host__ __device__ void agc_step(DevAgc &G, unsigned long long clipped, unsigned long long components,
                          unsigned long long epoch_after, DevAdjust *log, uint32_t cap) {
     G.clipped += clipped;
     G.components += components;
@@ -1072,6 +1074,126 @@ extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
     }
     batch_free(h);
     return status;
+}
+
+// ------------------------------------------------------------------ file sink
+// pharos-run's file sink (output/sink.zig:18-47) over the plain form: blocks of up to 100 epochs, the first one
+// ending on the 100-epoch grid.  Each block is synthesised on the device and copied to pinned host memory on a
+// second stream; a writer thread writes it to the file while the next block is synthesised.  One write is in flight
+// at a time, so blocks reach the file in order, and a slot's buffers are reused only after its write has finished.
+// The AGC runs between blocks on the host, as Run.pull does (engine/run.zig:225-256, gain.zig:19-35).
+struct SinkResources {
+    hipStream_t compute = nullptr, copy = nullptr;
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    uint8_t *d_ring = nullptr, *h_ring = nullptr;
+    FILE *file = nullptr;
+    ~SinkResources() {
+        if (copy) (void)hipStreamSynchronize(copy);
+        if (compute) (void)hipStreamSynchronize(compute);
+        for (hipEvent_t ev : copied) if (ev) (void)hipEventDestroy(ev);
+        if (compute) (void)hipStreamDestroy(compute);
+        if (copy) (void)hipStreamDestroy(copy);
+        if (d_ring) (void)hipFree(d_ring);
+        if (h_ring) (void)hipHostFree(h_ring);
+        if (file) fclose(file);
+    }
+};
+
+extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path, uint64_t *bytes_written) {
+    g_error[0] = 0;
+    if (bytes_written) *bytes_written = 0;
+    if (!J || !path) { set_error("null argument"); return ST_ARG; }
+    if (!J->engine || (J->channels && !J->rows) || J->epochs == 0) { set_error("the job is incomplete"); return ST_ARG; }
+    if (J->gain_mode > 1) { set_error("gain mode %u is not defined", J->gain_mode); return ST_ARG; }
+    if (J->gain_mode == PHAROS_SYNTH_GAIN_CONSTANT && !(J->constant_gain > 0.0f)) { set_error("constant gain must be positive"); return ST_ARG; }
+    if (J->gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE && !J->agc) { set_error("adaptive gain needs its AGC state"); return ST_ARG; }
+    pharos_synth_engine *e = J->engine;
+    const uint64_t bpe = e->view.bpe;
+    const size_t block_bytes = (size_t)100 * bpe;
+    HIP_TRY(hipSetDevice(e->device), ST_INTERNAL);
+    SinkResources R;
+    HIP_TRY(hipStreamCreate(&R.compute), ST_INTERNAL);
+    HIP_TRY(hipStreamCreate(&R.copy), ST_INTERNAL);
+    for (hipEvent_t &ev : R.copied) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), ST_INTERNAL);
+    if (hipMalloc((void **)&R.d_ring, 2 * block_bytes) != hipSuccess) { R.d_ring = nullptr; set_error("device memory for two blocks"); return ST_NOMEM; }
+    if (hipHostMalloc((void **)&R.h_ring, 2 * block_bytes, hipHostMallocDefault) != hipSuccess) { R.h_ring = nullptr; set_error("pinned memory for two blocks"); return ST_NOMEM; }
+    R.file = fopen(path, "wb");
+    if (!R.file) { set_error("cannot create %s: %s", path, strerror(errno)); return ST_ARG; }
+
+    DevAgc G{};
+    if (J->gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) {
+        G.gain = J->agc->gain;
+        G.clipped = J->agc->clipped;
+        G.components = J->agc->components;
+    }
+    std::vector<DevAdjust> log(std::max<uint32_t>(J->adjustments_cap, 1));
+    std::future<int> writing;  // errno of the block in flight, 0 when written
+    uint64_t written = 0, pending = 0, clip_total = 0;
+    int status = ST_OK, write_errno = 0;
+    uint64_t clip[100];
+    int k = 0;
+    for (uint64_t done = 0; done < J->epochs; k ^= 1) {
+        const uint64_t abs_e = J->first_epoch + done;
+        const uint32_t n = (uint32_t)std::min<uint64_t>(100 - abs_e % 100, J->epochs - done);
+        const float gain = J->gain_mode == PHAROS_SYNTH_GAIN_CONSTANT ? J->constant_gain : (float)G.gain;
+        uint8_t *d_slot = R.d_ring + (size_t)k * block_bytes, *h_slot = R.h_ring + (size_t)k * block_bytes;
+        status = pharos_synth_window(e, abs_e, n, J->channels, J->rows + done * J->channels, J->masks, J->mask_len,
+                                     gain, d_slot, clip, R.compute);  // returns with the block in d_slot
+        if (status) break;
+        if (writing.valid()) {  // the previous block's write; after it, this slot's buffers are free
+            write_errno = writing.get();
+            if (write_errno) break;
+            written += pending;
+        }
+        const size_t nbytes = (size_t)n * bpe;
+        if (hipMemcpyAsync(h_slot, d_slot, nbytes, hipMemcpyDeviceToHost, R.copy) != hipSuccess ||
+            hipEventRecord(R.copied[k], R.copy) != hipSuccess) {
+            set_error("block read-back failed");
+            status = ST_INTERNAL;
+            break;
+        }
+        hipEvent_t ev = R.copied[k];
+        FILE *f = R.file;
+        writing = std::async(std::launch::async, [ev, f, h_slot, nbytes]() -> int {
+            if (hipEventSynchronize(ev) != hipSuccess) return EIO;
+            return fwrite(h_slot, 1, nbytes, f) == nbytes ? 0 : (errno ? errno : EIO);
+        });
+        pending = nbytes;
+        uint64_t sum = 0;
+        for (uint32_t i = 0; i < n; i++) sum += clip[i];
+        clip_total += sum;
+        if (J->gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE)
+            agc_step(G, sum, (unsigned long long)n * e->view.spe * 2ull, abs_e + n, log.data(), (uint32_t)log.size());
+        done += n;
+    }
+    if (writing.valid()) {
+        const int w = writing.get();
+        if (!write_errno) write_errno = w;
+        if (!w) written += pending;
+    }
+    if (!write_errno && fflush(R.file) != 0) write_errno = errno ? errno : EIO;
+    if (bytes_written) *bytes_written = written;
+    if (J->clipped) *J->clipped = clip_total;
+    if (J->gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) {
+        const uint32_t prior = J->agc->adjustments;
+        J->agc->gain = G.gain;
+        J->agc->clipped = G.clipped;
+        J->agc->components = G.components;
+        J->agc->adjustments = prior + G.adjustments;
+        if (J->adjustments) {
+            const uint32_t n = std::min<uint32_t>(G.adjustments, std::min<uint32_t>(J->adjustments_cap, (uint32_t)log.size()));
+            for (uint32_t i = 0; i < n; i++) {
+                J->adjustments[i].epoch = log[i].epoch;
+                J->adjustments[i].gain = log[i].gain;
+                J->adjustments[i].clip_rate = log[i].rate;
+                J->adjustments[i].reduced = log[i].reduced;
+                J->adjustments[i].reserved = 0;
+            }
+        }
+    }
+    if (status) return status;
+    if (write_errno) { set_error("writing %s: %s", path, strerror(write_errno)); return ST_INTERNAL; }
+    return ST_OK;
 }
 
 extern "C" int pharos_synth_batch(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning, void *stream) {

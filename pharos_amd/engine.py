@@ -4,6 +4,7 @@ Device memory, streams and events come from the library itself (pharos_synth_mal
 runtime owns the device; every sample is computed by libpharos_amd.so's gfx950 kernels.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -217,6 +218,15 @@ class Job:
 
     def result(self):
         return self.out.to_numpy()
+
+    def to_file(self, path):
+        """The run written to `path` by the library's file sink (pharos_synth_to_file: blocks synthesised on the
+        device while the previous block is copied and written); returns the bytes on file.  The AGC state,
+        adjustments and clip total come back as after a batched run."""
+        n = C.c_uint64(0)
+        rec = self.record()
+        check(_lib.lib().pharos_synth_to_file(C.byref(rec), os.fsencode(str(path)), C.byref(n)))
+        return n.value
 
 
 def plain_job(job, stream=None, window=100):

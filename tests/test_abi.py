@@ -56,6 +56,8 @@ def test_argument_errors_need_no_device():
     assert b"null" in L.pharos_synth_last_error().lower() or L.pharos_synth_last_error()
     assert L.pharos_synth_window(None, 0, 1, 0, None, None, 0, C.c_float(1.0), None, None, None) == 2
     assert L.pharos_synth_batch_submit(None, 0, None, None, C.byref(out)) == 2
+    n = C.c_uint64(7)
+    assert L.pharos_synth_to_file(None, b"/nonexistent/x.bin", C.byref(n)) == 2 and n.value == 0
 
 
 C_PROBE = r"""
