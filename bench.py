@@ -232,6 +232,29 @@ def bench_single(args):
     return line
 
 
+def bench_sink(args):
+    """The file sink (pharos_synth_to_file) over the first --sink-epochs epochs of the same run: the rate with the
+    device-to-host copies and the file writes in, which the device-resident `value` leaves out."""
+    from pharos_amd import scenario as S
+    from pharos_amd.engine import Job
+    make = {"c0": S.c0, "c1": S.c1, "c2": S.c2}[args.workload]
+    sc = make(seed=1, epochs=args.sink_epochs)
+    rows, masks = sc.window(0, sc.epochs)
+    job = Job(sc, rows=rows, masks=masks)
+    path = os.path.join(args.sink, f"pharos_sink_{os.getpid()}.bin")
+    try:
+        job.to_file(path)  # warm-up: the pinned and device rings, the file
+        t0 = time.perf_counter()
+        n = job.to_file(path)
+        dt = time.perf_counter() - t0
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+    return {"value": sc.channel_samples / dt, "unit": "channel-samples/s", "bytes": n, "gb_per_s": n / dt / 1e9,
+            "epochs": sc.epochs, "seconds": dt,
+            "form": "100-epoch plain blocks, device-to-pinned copies and file writes overlapped with synthesis"}
+
+
 def bench_c3(args):
     from concurrent.futures import ThreadPoolExecutor
     from pharos_amd import scenario as S
@@ -298,6 +321,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window")
     ap.add_argument("--serve-epochs-cap", type=int, default=0, help="longest request in epochs (0: up to 30 s)")
+    ap.add_argument("--sink", metavar="DIR", default="",
+                    help="also time the file sink: the first --sink-epochs epochs of the run written to a file in DIR")
+    ap.add_argument("--sink-epochs", type=int, default=6000)
     ap.add_argument("--workgroups-per-cu", type=int, default=0, help="launch shape (skips the autotuner)")
     ap.add_argument("--channels-per-pass", type=int, default=0, help="launch shape (skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true", help="use the library's default launch shape")
@@ -322,6 +348,8 @@ def main():
         return
 
     line = bench_c3(args) if args.workload == "c3" else bench_single(args)
+    if args.sink and args.workload != "c3":
+        line["sink"] = bench_sink(args)
     if not args.no_serve:
         from pharos_amd import serve
         line["serving"] = serve.measure(duration_s=args.serve_seconds, rate_hz=args.serve_rate,

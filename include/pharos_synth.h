@@ -176,10 +176,11 @@ void pharos_synth_graph_destroy(pharos_synth_graph *graph);
 
 /* The job's run written to the file at `path`, as pharos-run's file sink does (output/sink.zig:18-47, behind
  * pharos_run_to_file): blocks of up to 100 epochs, the first ending on the 100-epoch grid, each synthesised on the
- * device while the previous one is copied to pinned host memory and written by a writer thread.  The AGC runs
- * between blocks as Run.pull does, and the job's agc, adjustments and clipped receive the results as after
- * pharos_synth_batch_wait.  job->out is not used (the sink holds two blocks of device and pinned memory).
- * *bytes_written is the number of bytes on file; a write error gives status 4 and its message. */
+ * device while earlier ones are copied to pinned host memory and written at their own file offsets by writer
+ * threads.  The AGC runs between blocks as Run.pull does, and the job's agc, adjustments and clipped receive the
+ * results as after pharos_synth_batch_wait.  job->out is not used (the sink holds four blocks of device and pinned
+ * memory).  *bytes_written is the file's length when every write succeeded; a write error gives status 4, its
+ * message and 0. */
 int pharos_synth_to_file(const pharos_synth_job *job, const char *path, uint64_t *bytes_written);
 
 /* Device plumbing for callers without a HIP runtime of their own (the library's runtime owns the device).

@@ -11,6 +11,8 @@
 #include <stdio.h>
 #include <string.h>
 #include <errno.h>
+#include <fcntl.h>
+#include <unistd.h>
 #include <algorithm>
 #include <future>
 #include <mutex>
@@ -1079,14 +1081,17 @@ extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
 // ------------------------------------------------------------------ file sink
 // pharos-run's file sink (output/sink.zig:18-47) over the plain form: blocks of up to 100 epochs, the first one
 // ending on the 100-epoch grid.  Each block is synthesised on the device and copied to pinned host memory on a
-// second stream; a writer thread writes it to the file while the next block is synthesised.  One write is in flight
-// at a time, so blocks reach the file in order, and a slot's buffers are reused only after its write has finished.
-// The AGC runs between blocks on the host, as Run.pull does (engine/run.zig:225-256, gain.zig:19-35).
+// second stream; a writer thread writes it at its own file offset (pwrite) while the next blocks are synthesised.
+// SINK_SLOTS blocks can be in flight, so several writes copy into the page cache at once; a slot's buffers are
+// reused only after its write has finished.  The AGC runs between blocks on the host, as Run.pull does
+// (engine/run.zig:225-256, gain.zig:19-35).
+constexpr int SINK_SLOTS = 4;
+
 struct SinkResources {
     hipStream_t compute = nullptr, copy = nullptr;
-    hipEvent_t copied[2] = {nullptr, nullptr};
+    hipEvent_t copied[SINK_SLOTS] = {};
     uint8_t *d_ring = nullptr, *h_ring = nullptr;
-    FILE *file = nullptr;
+    int fd = -1;
     ~SinkResources() {
         if (copy) (void)hipStreamSynchronize(copy);
         if (compute) (void)hipStreamSynchronize(compute);
@@ -1095,9 +1100,25 @@ struct SinkResources {
         if (copy) (void)hipStreamDestroy(copy);
         if (d_ring) (void)hipFree(d_ring);
         if (h_ring) (void)hipHostFree(h_ring);
-        if (file) fclose(file);
+        if (fd >= 0) close(fd);
     }
 };
+
+// Writes all of buf at offset; 0 or the errno of the failure.
+static int write_at(int fd, const uint8_t *buf, size_t n, uint64_t offset) {
+    while (n) {
+        const ssize_t w = pwrite(fd, buf, n, (off_t)offset);
+        if (w < 0) {
+            if (errno == EINTR) continue;
+            return errno;
+        }
+        if (w == 0) return EIO;
+        buf += w;
+        n -= (size_t)w;
+        offset += (uint64_t)w;
+    }
+    return 0;
+}
 
 extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path, uint64_t *bytes_written) {
     g_error[0] = 0;
@@ -1115,10 +1136,18 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
     HIP_TRY(hipStreamCreate(&R.compute), ST_INTERNAL);
     HIP_TRY(hipStreamCreate(&R.copy), ST_INTERNAL);
     for (hipEvent_t &ev : R.copied) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), ST_INTERNAL);
-    if (hipMalloc((void **)&R.d_ring, 2 * block_bytes) != hipSuccess) { R.d_ring = nullptr; set_error("device memory for two blocks"); return ST_NOMEM; }
-    if (hipHostMalloc((void **)&R.h_ring, 2 * block_bytes, hipHostMallocDefault) != hipSuccess) { R.h_ring = nullptr; set_error("pinned memory for two blocks"); return ST_NOMEM; }
-    R.file = fopen(path, "wb");
-    if (!R.file) { set_error("cannot create %s: %s", path, strerror(errno)); return ST_ARG; }
+    if (hipMalloc((void **)&R.d_ring, SINK_SLOTS * block_bytes) != hipSuccess) {
+        R.d_ring = nullptr;
+        set_error("device memory for %d blocks", SINK_SLOTS);
+        return ST_NOMEM;
+    }
+    if (hipHostMalloc((void **)&R.h_ring, SINK_SLOTS * block_bytes, hipHostMallocDefault) != hipSuccess) {
+        R.h_ring = nullptr;
+        set_error("pinned memory for %d blocks", SINK_SLOTS);
+        return ST_NOMEM;
+    }
+    R.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (R.fd < 0) { set_error("cannot create %s: %s", path, strerror(errno)); return ST_ARG; }
 
     DevAgc G{};
     if (J->gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) {
@@ -1127,24 +1156,23 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
         G.components = J->agc->components;
     }
     std::vector<DevAdjust> log(std::max<uint32_t>(J->adjustments_cap, 1));
-    std::future<int> writing;  // errno of the block in flight, 0 when written
-    uint64_t written = 0, pending = 0, clip_total = 0;
+    std::future<int> writing[SINK_SLOTS];  // per slot: errno of its block's write, 0 when written
+    uint64_t clip_total = 0, offset = 0;
     int status = ST_OK, write_errno = 0;
     uint64_t clip[100];
-    int k = 0;
-    for (uint64_t done = 0; done < J->epochs; k ^= 1) {
+    for (uint64_t done = 0, block = 0; done < J->epochs; block++) {
+        const int k = (int)(block % SINK_SLOTS);
         const uint64_t abs_e = J->first_epoch + done;
         const uint32_t n = (uint32_t)std::min<uint64_t>(100 - abs_e % 100, J->epochs - done);
         const float gain = J->gain_mode == PHAROS_SYNTH_GAIN_CONSTANT ? J->constant_gain : (float)G.gain;
         uint8_t *d_slot = R.d_ring + (size_t)k * block_bytes, *h_slot = R.h_ring + (size_t)k * block_bytes;
+        if (writing[k].valid()) {  // the slot's previous block: after its write, its buffers are free
+            write_errno = writing[k].get();
+            if (write_errno) break;
+        }
         status = pharos_synth_window(e, abs_e, n, J->channels, J->rows + done * J->channels, J->masks, J->mask_len,
                                      gain, d_slot, clip, R.compute);  // returns with the block in d_slot
         if (status) break;
-        if (writing.valid()) {  // the previous block's write; after it, this slot's buffers are free
-            write_errno = writing.get();
-            if (write_errno) break;
-            written += pending;
-        }
         const size_t nbytes = (size_t)n * bpe;
         if (hipMemcpyAsync(h_slot, d_slot, nbytes, hipMemcpyDeviceToHost, R.copy) != hipSuccess ||
             hipEventRecord(R.copied[k], R.copy) != hipSuccess) {
@@ -1152,13 +1180,14 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
             status = ST_INTERNAL;
             break;
         }
-        hipEvent_t ev = R.copied[k];
-        FILE *f = R.file;
-        writing = std::async(std::launch::async, [ev, f, h_slot, nbytes]() -> int {
+        const hipEvent_t ev = R.copied[k];
+        const int fd = R.fd;
+        const uint64_t at = offset;
+        writing[k] = std::async(std::launch::async, [ev, fd, h_slot, nbytes, at]() -> int {
             if (hipEventSynchronize(ev) != hipSuccess) return EIO;
-            return fwrite(h_slot, 1, nbytes, f) == nbytes ? 0 : (errno ? errno : EIO);
+            return write_at(fd, h_slot, nbytes, at);
         });
-        pending = nbytes;
+        offset += nbytes;
         uint64_t sum = 0;
         for (uint32_t i = 0; i < n; i++) sum += clip[i];
         clip_total += sum;
@@ -1166,13 +1195,13 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
             agc_step(G, sum, (unsigned long long)n * e->view.spe * 2ull, abs_e + n, log.data(), (uint32_t)log.size());
         done += n;
     }
-    if (writing.valid()) {
-        const int w = writing.get();
-        if (!write_errno) write_errno = w;
-        if (!w) written += pending;
-    }
-    if (!write_errno && fflush(R.file) != 0) write_errno = errno ? errno : EIO;
-    if (bytes_written) *bytes_written = written;
+    for (auto &w : writing)
+        if (w.valid()) {
+            const int r = w.get();
+            if (!write_errno) write_errno = r;
+        }
+    // The bytes on file: everything submitted, when every write succeeded (a failed write leaves a hole).
+    if (bytes_written) *bytes_written = write_errno ? 0 : offset;
     if (J->clipped) *J->clipped = clip_total;
     if (J->gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) {
         const uint32_t prior = J->agc->adjustments;
