@@ -183,6 +183,13 @@ void pharos_synth_graph_destroy(pharos_synth_graph *graph);
  * message and 0. */
 int pharos_synth_to_file(const pharos_synth_job *job, const char *path, uint64_t *bytes_written);
 
+/* The .tag sidecar of a raw sample file (output/tag.zig:12-34, behind pharos_run_tag_text): seven lines naming the
+ * program, the GPS calendar time of the first sample, CS8 or CS16, the sample rate and centre frequency in MHz
+ * (six decimals), IQ = 2 and the bits per component.  *len receives the text's length; the text is copied when buf
+ * holds it.  IQ2 and IQ4 have no .tag form (status 2), nor do a null buf or a short one. */
+int pharos_synth_tag_text(uint32_t format, uint64_t sample_rate_hz, int64_t center_hz, int64_t first_gps_ms,
+                          char *buf, size_t cap, size_t *len);
+
 /* Device plumbing for callers without a HIP runtime of their own (the library's runtime owns the device).
  * Copies are synchronous; kind 0 host to device, 1 device to host, 2 device to device. */
 int pharos_synth_device_count(void);

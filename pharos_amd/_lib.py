@@ -108,6 +108,7 @@ def lib():
         "pharos_synth_batch_query": (C.c_int, [VP]),
         "pharos_synth_batch_wait": (C.c_int, [VP]),
         "pharos_synth_to_file": (C.c_int, [P(SynthJob), C.c_char_p, P(U64)]),
+        "pharos_synth_tag_text": (C.c_int, [U32, U64, I64, I64, C.c_char_p, C.c_size_t, P(C.c_size_t)]),
         "pharos_synth_graph_create": (C.c_int, [P(SynthJob), P(VP)]),
         "pharos_synth_graph_launch": (C.c_int, [VP, VP]),
         "pharos_synth_graph_finish": (C.c_int, [VP, VP]),
