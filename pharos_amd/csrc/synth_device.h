@@ -54,6 +54,14 @@ struct DevEngine {
     uint32_t pad;
 };
 
+// Per row (epoch, channel), computed once for all of the row's tiles by the batched form's pre-pass: the lane powers
+// of Rotor.init (rotor.zig:21-30) and the f32 16-step rotation w16 (rotor.zig:31-33).
+struct RowPrep {
+    double pw_c[LANES16], pw_s[LANES16];
+    float w_re, w_im;
+    uint32_t pad[2];
+};  // 272 B
+
 // Per channel, per tile (LDS).
 struct ChanRec {
     double pw_c[LANES16], pw_s[LANES16];  // w1^l, the rotor's f64 lane powers (rotor.zig:25-30)
@@ -93,6 +101,7 @@ struct DevJob {
     uint32_t gain_mode, first_segment;
     float constant_gain;
     uint32_t segments;
+    const RowPrep *prep;       // [epochs][channels]
 };
 
 struct DevSegment {

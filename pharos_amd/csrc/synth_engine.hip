@@ -75,6 +75,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngin
     A.epoch = first_epoch + e;
     A.out_epoch = out + (size_t)e * E->bpe;
     A.chunk_cap = 0;
+    A.prep = nullptr;
     // The gain is the argument, or a device word another kernel of the same stream wrote (the graph form).
     uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain_dev ? *gmem(gain_dev) : gain; }, []() {}, err);
     if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
@@ -206,6 +207,18 @@ __global__ __launch_bounds__(AGC_THREADS) void agc_window_kernel(DevAgc *G, cons
     }
 }
 
+// The batched form's pre-pass: RowPrep of every active row of a job, one thread per row.
+__global__ __launch_bounds__(256) void row_prep_kernel(const pharos_synth_row *rows, uint64_t n, const double *turn,
+                                                       RowPrep *out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const pharos_synth_row row = rows[i];
+    if (row.table == PHAROS_SYNTH_INACTIVE) return;
+    RowPrep p;
+    row_prep(row, turn, p);
+    out[i] = p;
+}
+
 template <uint32_t F>
 __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -253,6 +266,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
         A.epoch = J.first_epoch + e_off;
         A.out_epoch = J.out + (size_t)e_off * J.engine->bpe;
         A.chunk_cap = S.chunk_cap;
+        A.prep = J.prep ? J.prep + (size_t)e_off * J.channels : nullptr;
         auto gain_fn = [&]() -> float {
             if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT) return J.constant_gain;
             if (threadIdx.x == 0) {
@@ -913,6 +927,16 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
           p_err = carve(64);
     const size_t image = total;
     Piece p_adj = carve(sizeof(DevAdjust) * (size_t)adjust_cap * njobs);
+    // The pre-pass serves jobs whose epochs span several tiles; with one tile per epoch a row's constants are used
+    // once, and the tile computes them itself.
+    std::vector<size_t> prep_at(njobs, SIZE_MAX);
+    size_t prep_rows = 0;
+    for (uint32_t j = 0; j < njobs; j++) {
+        if (jobs[j].engine->view.tiles < 2) continue;
+        prep_at[j] = prep_rows;
+        prep_rows += (size_t)jobs[j].epochs * jobs[j].channels;
+    }
+    Piece p_prep = carve(sizeof(RowPrep) * std::max<size_t>(prep_rows, 1));
     h->pinned = pinned_take(image, &h->pinned_bytes);
     if (!h->pinned) { set_error("hipHostMalloc(%zu) failed", image); batch_free(h); return ST_NOMEM; }
     h->at_agc = p_agc.off;
@@ -943,6 +967,9 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     int st = batch_alloc(h, &base, total);
     if (st) { batch_free(h); return st; }
     uint8_t *b8 = (uint8_t *)base;
+    for (uint32_t j = 0; j < njobs; j++)
+        reinterpret_cast<DevJob *>(host + p_jobs.off)[j].prep =
+            prep_at[j] == SIZE_MAX ? nullptr : reinterpret_cast<const RowPrep *>(b8 + p_prep.off) + prep_at[j];
     if (hipMemcpyAsync(base, host, image, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
         set_error("batch upload failed");
         batch_free(h);
@@ -969,6 +996,19 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     h->d_seg_count = S.seg_count;
     h->d_err = S.err;
 
+    // The pre-pass: each job's per-row rotor constants, once for all of a row's tiles.
+    for (uint32_t j = 0; j < njobs; j++) {
+        const uint64_t n = (uint64_t)jobs[j].epochs * jobs[j].channels;
+        if (n == 0 || prep_at[j] == SIZE_MAX) continue;
+        hipLaunchKernelGGL(row_prep_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, h->stream, jobs[j].rows, n,
+                           jobs[j].engine->view.turn, reinterpret_cast<RowPrep *>(b8 + p_prep.off) + prep_at[j]);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("pre-pass launch failed");
+            (void)hipStreamSynchronize(h->stream);
+            batch_free(h);
+            return ST_INTERNAL;
+        }
+    }
     {
         const uint32_t lds = S.queue_words_offset + 16;
         const void *fn = nullptr;

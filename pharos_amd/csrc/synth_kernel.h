@@ -296,6 +296,29 @@ __device__ __forceinline__ void stage_window(uint32_t *dst, const uint32_t *src_
     }
 }
 
+// The per-row part of Rotor.init (rotor.zig:21-33) for one row: the f64 lane powers by the reference's recursion
+// and f32 of w16, exactly as chan_group computes them.
+__device__ __forceinline__ void row_prep(const pharos_synth_row &row, const double *turn, RowPrep &out) {
+    double w1c, w1s, wc, ws;
+    sincos64(row.carrier_step, turn, w1c, w1s);
+    sincos64(16.0 * row.carrier_step, turn, wc, ws);
+    double ac = 1.0, as = 0.0;
+    out.pw_c[0] = 1.0;
+    out.pw_s[0] = 0.0;
+#pragma unroll
+    for (uint32_t l = 1; l < LANES16; l++) {
+        const double nc = ac * w1c - as * w1s;
+        const double ns = ac * w1s + as * w1c;
+        ac = nc;
+        as = ns;
+        out.pw_c[l] = ac;
+        out.pw_s[l] = as;
+    }
+    out.w_re = (float)wc;
+    out.w_im = (float)ws;
+    out.pad[0] = out.pad[1] = 0;
+}
+
 // Setup and staging of one channel for one tile, by its group of G threads (thread j of the group).  Every thread
 // of the group derives the channel's code edge, staged window, shape and symbol masks from the row (in LDS) in
 // registers, so the group stages the window with no barrier between; then thread j writes
@@ -307,7 +330,7 @@ __device__ __forceinline__ void stage_window(uint32_t *dst, const uint32_t *src_
 // one 4.
 template <uint32_t G>
 __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *stage, uint32_t stage_addr,
-                                           const pharos_synth_row &row,
+                                           const pharos_synth_row &row, const RowPrep *prep,
                                            const DevEngine &E, uint32_t n0, uint32_t tile_len, const uint8_t *masks,
                                            uint64_t mask_len, uint32_t *err, uint32_t j) {
     const bool first_lane = j == 0;
@@ -333,9 +356,18 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
         vre[i] = k0 + i < 8 ? T.vre[k0 + i] : 0.0;
         vim[i] = k0 + i < 8 ? T.vim[k0 + i] : 0.0;
     }
-    double w1c, w1s, wc, ws;
-    sincos64(row.carrier_step, E.turn, w1c, w1s);
-    sincos64(16.0 * row.carrier_step, E.turn, wc, ws);
+    double w1c = 0.0, w1s = 0.0;
+    float w_re, w_im;
+    if (prep) {
+        w_re = gmem(prep)->w_re;
+        w_im = gmem(prep)->w_im;
+    } else {
+        double wc, ws;
+        sincos64(row.carrier_step, E.turn, w1c, w1s);
+        sincos64(16.0 * row.carrier_step, E.turn, wc, ws);
+        w_re = (float)wc;
+        w_im = (float)ws;
+    }
     if (!(row.entries_per_sample > 0.0) || !(row.entries_per_sample < 64.0) || !(row.entry_phase0 >= 0.0)) {
         if (first_lane) atomicOr(err, ERR_TABLE);
         return;
@@ -395,21 +427,30 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
     for (uint32_t i = 0; i < PER; i++)
         vt[k0 + i] = k0 + i < count ? make_float2((float)((double)row.amp * vre[i]), (float)((double)row.amp * vim[i]))
                                     : make_float2(0.0f, 0.0f);
-    // This thread's lane powers of the recursion pw[l] = pw[l-1] * w1 (rotor.zig:25-30).
-    double ac = 1.0, as = 0.0;
-    if (first_lane) {
-        R.pw_c[0] = 1.0;
-        R.pw_s[0] = 0.0;
-    }
+    // This thread's lane powers of the recursion pw[l] = pw[l-1] * w1 (rotor.zig:25-30), from the row's pre-pass
+    // when there is one.
+    if (prep) {
 #pragma unroll
-    for (uint32_t l = 1; l < LANES16; l++) {
-        const double nc = ac * w1c - as * w1s;
-        const double ns = ac * w1s + as * w1c;
-        ac = nc;
-        as = ns;
-        if (l >= k0 && l < k0 + PER) {
-            R.pw_c[l] = ac;
-            R.pw_s[l] = as;
+        for (uint32_t i = 0; i < PER; i++) {
+            R.pw_c[k0 + i] = gmem(prep)->pw_c[k0 + i];
+            R.pw_s[k0 + i] = gmem(prep)->pw_s[k0 + i];
+        }
+    } else {
+        double ac = 1.0, as = 0.0;
+        if (first_lane) {
+            R.pw_c[0] = 1.0;
+            R.pw_s[0] = 0.0;
+        }
+#pragma unroll
+        for (uint32_t l = 1; l < LANES16; l++) {
+            const double nc = ac * w1c - as * w1s;
+            const double ns = ac * w1s + as * w1c;
+            ac = nc;
+            as = ns;
+            if (l >= k0 && l < k0 + PER) {
+                R.pw_c[l] = ac;
+                R.pw_s[l] = as;
+            }
         }
     }
     // Code positions: lane offsets l * eps_q and block starts P0 + 512 b eps_q by repeated addition; the block
@@ -432,8 +473,8 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
         R.lead_q = lead_q;
         R.lead_whole = lead_whole;
         R.lead_bits = row.lead_bits;
-        R.w_re = (float)wc;
-        R.w_im = (float)ws;
+        R.w_re = w_re;
+        R.w_im = w_im;
         R.active = 1;
     }
 }
@@ -676,6 +717,7 @@ struct TileArgs {
     uint64_t epoch;                // absolute epoch number
     uint8_t *out_epoch;            // this epoch's slot
     uint32_t chunk_cap;            // channels per LDS pass at most (0: the engine's)
+    const RowPrep *prep;           // this epoch's rows' pre-pass, or null (computed per tile)
 };
 
 
@@ -722,15 +764,15 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
             if (c < nc)
                 chan_group<8>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
                               reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot),
-                              smem_lds + L.stage_base + c * E.w_slot, L.rows[c], E, n0, tile_len, A.masks, A.mask_len,
-                              err, j);
+                              smem_lds + L.stage_base + c * E.w_slot, L.rows[c], A.prep ? A.prep + c0 + c : nullptr,
+                              E, n0, tile_len, A.masks, A.mask_len, err, j);
         } else {
             const uint32_t c = tid / GROUP, j = tid % GROUP;
             if (c < nc)
                 chan_group<GROUP>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
                                   reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot),
-                                  smem_lds + L.stage_base + c * E.w_slot, L.rows[c], E, n0, tile_len, A.masks,
-                                  A.mask_len, err, j);
+                                  smem_lds + L.stage_base + c * E.w_slot, L.rows[c],
+                                  A.prep ? A.prep + c0 + c : nullptr, E, n0, tile_len, A.masks, A.mask_len, err, j);
         }
         __syncthreads();
         // The next chunk's rows travel from global memory while this chunk computes.
