@@ -789,7 +789,9 @@ static int batch_alloc(pharos_synth_batch_handle *h, void **p, size_t n) {
             }
         }
     }
-    const size_t bytes = std::max(n, SPARE_MIN_BYTES);
+    // Sizes round up to a power of two, so that a serving loop's batches of varying size reuse cached blocks.
+    size_t bytes = SPARE_MIN_BYTES;
+    while (bytes < n) bytes *= 2;
     hipError_t st = hipMalloc(p, bytes);
     if (st != hipSuccess) { set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(st)); return ST_NOMEM; }
     h->owned.push_back(*p);
