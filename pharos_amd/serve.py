@@ -29,7 +29,8 @@ from .engine import Engine, DeviceBuffer, Stream, require_gpu
 
 SLICE_MIN = 1000        # epochs of a request per iteration, at least (ten AGC windows) ...
 SLICE_WORK = 8e9        # ... or about this many channel-samples (a few ms of the GPU), in whole AGC windows
-RESERVE = 4 << 30       # HBM left free for the library's own allocations
+RESERVE = 24 << 30      # HBM left free for the library's own allocations: engines, and the batch state it caches
+                        # (up to four blocks, each the power of two above a batch's tables and row pre-pass)
 
 
 class Request:
