@@ -214,6 +214,8 @@ def bench_single(args):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "valu_busy_pct_profiled": valu_busy,
                      "kernel": "synth_batch_kernel", "kernel_ms": kernel_s * 1e3,
+                     "kernel_ms_spans": "row_prep_kernel pre-pass + synth_batch_kernel + batch-table copies, HIP events "
+                                        "on the launch stream",
                      "flops_per_launch": flops_per_step, "hbm_write_gbs": out_bytes / kernel_s / 1e9,
                      "hbm_write_frac": out_bytes / kernel_s / 1e9 / PEAK_HBM_GBS},
         "graph": {"value": cs_per_step / (graph_ms / 1e3) if graph_ms else None, "unit": "channel-samples/s",
