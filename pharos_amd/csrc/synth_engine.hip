@@ -62,7 +62,8 @@ __global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngin
                                                                uint32_t channels, const uint8_t *__restrict__ masks,
                                                                uint64_t mask_len, uint64_t first_epoch, float gain,
                                                                uint8_t *__restrict__ out, uint32_t *__restrict__ clip_partial,
-                                                               uint32_t *err, const float *gain_dev) {
+                                                               uint32_t *err, const float *gain_dev,
+                                                               const RowPrep *prep) {
     extern __shared__ __align__(16) uint8_t smem[];
     const uint32_t tile = blockIdx.x, e = blockIdx.y;
     TileArgs A;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngin
     A.epoch = first_epoch + e;
     A.out_epoch = out + (size_t)e * E->bpe;
     A.chunk_cap = 0;
-    A.prep = nullptr;
+    A.prep = prep ? prep + (size_t)e * channels : nullptr;
     // The gain is the argument, or a device word another kernel of the same stream wrote (the graph form).
     uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain_dev ? *gmem(gain_dev) : gain; }, []() {}, err);
     if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
@@ -682,7 +683,7 @@ extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch,
 #define PHAROS_WINDOW(FMT)                                                                                         \
         case FMT:                                                                                                \
             hipLaunchKernelGGL(synth_window_kernel<FMT>, grid, dim3(THREADS), lds, stream, e->d_engine, r, channels, \
-                               masks, mask_len, first_epoch + done, gain, o, cp, e->d_err, nullptr);            \
+                               masks, mask_len, first_epoch + done, gain, o, cp, e->d_err, nullptr, nullptr);   \
             break;
         PHAROS_WINDOW(PHAROS_SYNTH_IQ2)
         PHAROS_WINDOW(PHAROS_SYNTH_IQ4)
@@ -1304,13 +1305,13 @@ static void graph_free(pharos_synth_graph *g) {
 
 static void launch_window(pharos_synth_engine *e, const dim3 &grid, hipStream_t stream, const pharos_synth_row *rows,
                           uint32_t channels, const uint8_t *masks, uint64_t mask_len, uint64_t first_epoch, float gain,
-                          uint8_t *out, uint32_t *clip, const float *gain_dev) {
+                          uint8_t *out, uint32_t *clip, const float *gain_dev, const RowPrep *prep) {
     const uint32_t lds = e->view.lds_bytes;
     switch (e->view.format) {
 #define PHAROS_GRAPH_WINDOW(FMT)                                                                                   \
     case FMT:                                                                                                      \
         hipLaunchKernelGGL(synth_window_kernel<FMT>, grid, dim3(THREADS), lds, stream, e->d_engine, rows, channels, \
-                           masks, mask_len, first_epoch, gain, out, clip, e->d_err, gain_dev);                      \
+                           masks, mask_len, first_epoch, gain, out, clip, e->d_err, gain_dev, prep);                \
         break;
     PHAROS_GRAPH_WINDOW(PHAROS_SYNTH_IQ2)
     PHAROS_GRAPH_WINDOW(PHAROS_SYNTH_IQ4)
@@ -1342,7 +1343,10 @@ extern "C" int pharos_synth_graph_create(const pharos_synth_job *job, pharos_syn
     g->nclip = (size_t)J.epochs * tiles;
     size_t off_agc = 0, off_init = 256, off_gain = 512, off_adj = 768;
     size_t off_clip = (off_adj + sizeof(DevAdjust) * g->cap + 255) & ~(size_t)255;
-    size_t total = off_clip + g->nclip * sizeof(uint32_t);
+    // The row pre-pass (RowPrep per row), captured as the graph's first node, when an epoch spans several tiles.
+    const bool prepped = tiles >= 2 && J.channels > 0;
+    size_t off_prep = (off_clip + g->nclip * sizeof(uint32_t) + 255) & ~(size_t)255;
+    size_t total = off_prep + (prepped ? sizeof(RowPrep) * (size_t)J.epochs * J.channels : 0);
     if (hipMalloc(&g->mem, total) != hipSuccess) { set_error("hipMalloc(%zu) failed", total); graph_free(g); return ST_NOMEM; }
     uint8_t *b = (uint8_t *)g->mem;
     g->d_agc = (DevAgc *)(b + off_agc);
@@ -1350,6 +1354,7 @@ extern "C" int pharos_synth_graph_create(const pharos_synth_job *job, pharos_syn
     g->d_gain = (float *)(b + off_gain);
     g->d_adjust = (DevAdjust *)(b + off_adj);
     g->d_clip = (uint32_t *)(b + off_clip);
+    RowPrep *prep = prepped ? (RowPrep *)(b + off_prep) : nullptr;
 
     hipStream_t cs;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) { set_error("stream create failed"); graph_free(g); return ST_INTERNAL; }
@@ -1358,6 +1363,11 @@ extern "C" int pharos_synth_graph_create(const pharos_synth_job *job, pharos_syn
     hipError_t st = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (st == hipSuccess) {
         if (adaptive) hipLaunchKernelGGL(agc_begin_kernel, dim3(1), dim3(64), 0, cs, g->d_agc, g->d_initial, g->d_gain);
+        if (prepped) {
+            const uint64_t n = (uint64_t)J.epochs * J.channels;
+            hipLaunchKernelGGL(row_prep_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, cs, J.rows, n,
+                               e->view.turn, prep);
+        }
         uint32_t off = 0;
         while (off < J.epochs) {
             const uint64_t abs_e = J.first_epoch + off;
@@ -1367,7 +1377,7 @@ extern "C" int pharos_synth_graph_create(const pharos_synth_job *job, pharos_syn
                 launch_window(e, dim3(tiles, m), cs, J.rows + (size_t)(off + done) * J.channels, J.channels, J.masks,
                               J.mask_len, abs_e + done, adaptive ? 1.0f : J.constant_gain,
                               J.out + (size_t)(off + done) * bpe, g->d_clip + (size_t)(off + done) * tiles,
-                              adaptive ? g->d_gain : nullptr);
+                              adaptive ? g->d_gain : nullptr, prep ? prep + (size_t)(off + done) * J.channels : nullptr);
                 done += m;
             }
             if (adaptive)
