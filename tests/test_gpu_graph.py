@@ -53,3 +53,21 @@ def test_graph_constant_gain_from_a_mid_run_epoch():
         assert np.array_equal(job.result(), pb), replay
         assert job.clipped.value == int(pc.sum())
     g.close()
+
+
+def test_graph_with_the_row_pre_pass_from_a_mid_run_epoch():
+    """An engine with several tiles per epoch: the graph's first node computes the rows' rotor constants, and the
+    window launches read them at their own row offsets."""
+    sc = _strong_iq4(7, epochs=420)
+    print("case graph, strong IQ4 (4 tiles per epoch) from epoch 57", flush=True)
+    pb, pc, pgain, padj = run_plain(sc, first_epoch=57, epochs=363)
+    rows, masks = sc.window(57, 363)
+    job = Job(sc, first_epoch=57, epochs=363, rows=rows, masks=masks)
+    g = Graph(job)
+    for replay in range(2):
+        _reset(job)
+        g.run()
+        assert np.array_equal(job.result(), pb), replay
+        assert job.clipped.value == int(pc.sum()), replay
+        assert job.adjustments() == padj and job.agc.gain == pgain, replay
+    g.close()
