@@ -3,8 +3,11 @@
 // Plain form: pharos_synth_window launches one workgroup per (epoch, tile) of a window under one gain -- the
 // GPU replacement of Run.runJob(.produce) fanning synthSubTile out over a thread pool (engine/run.zig:277-296,
 // 367-380, engine/workers.zig:52-71).
-// Batched form: pharos_synth_batch runs a persistent kernel over a queue of (job, window, tile) items; the
-// adaptive gain (synthesis/gain.zig:14-35, applied by engine/run.zig:238-247) is carried on the device.
+// Batched form: pharos_synth_batch runs a persistent kernel over a queue of (job, window, tile) items, after a
+// pre-pass that computes each row's rotor constants once for all its tiles; the adaptive gain
+// (synthesis/gain.zig:14-35, applied by engine/run.zig:238-247) is carried on the device.
+// Graph form: a run's window launches, with the AGC step between them on the device, captured as one HIP graph.
+// File sink: a run written to disk in 100-epoch blocks (output/sink.zig:18-47).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
