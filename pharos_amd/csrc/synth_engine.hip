@@ -1229,10 +1229,16 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
         const hipEvent_t ev = R.copied[k];
         const int fd = R.fd;
         const uint64_t at = offset;
-        writing[k] = std::async(std::launch::async, [ev, fd, h_slot, nbytes, at]() -> int {
+        auto write_block = [ev, fd, h_slot, nbytes, at]() -> int {
             if (hipEventSynchronize(ev) != hipSuccess) return EIO;
             return write_at(fd, h_slot, nbytes, at);
-        });
+        };
+        try {
+            writing[k] = std::async(std::launch::async, write_block);
+        } catch (...) {  // no thread to be had: write this block here
+            write_errno = write_block();
+            if (write_errno) break;
+        }
         offset += nbytes;
         uint64_t sum = 0;
         for (uint32_t i = 0; i < n; i++) sum += clip[i];
