@@ -80,3 +80,39 @@ def test_every_format_in_one_launch_each_as_alone():
         assert j.clipped.value == int(pc.sum()), s.name
         if s.gain_db is None:
             assert j.adjustments() == padj and j.agc.gain == pgain, s.name
+
+
+def _batch_vs_oracle(sc, rows, masks, first=0):
+    from tests.helpers import oracle_window, describe_mismatch
+    epochs = rows.shape[0]
+    job = Job(sc, first_epoch=first, epochs=epochs, rows=rows, masks=masks)
+    (job,) = run_batch([job])
+    ob, oc, _, _ = oracle_window(sc, first, epochs, 1.0, rows=rows, masks=masks)
+    got = job.result()
+    assert np.array_equal(got, ob), describe_mismatch(got, ob, "bytes")
+    assert job.clipped.value == int(oc.sum())
+
+
+def test_batch_pre_pass_with_inactive_rows():
+    """Several tiles per epoch (the row pre-pass) with inactive rows, against the oracle."""
+    from tests.test_gpu_signals import _channels
+    from pharos_amd import _lib
+    sc = Scenario("inactive_batch", 16_368, "IQ8", 3, _channels("L1CA", 5, 8, 16_368), noise=True, seed=19,
+                  gain_db=0.0)
+    print("case batch, 5 L1CA channels at 16.368 Msps with inactive rows", flush=True)
+    rows, masks = sc.window(0, 3)
+    rows["table"][1, 2] = _lib.INACTIVE
+    rows["table"][:, 4] = _lib.INACTIVE
+    _batch_vs_oracle(sc, rows, masks)
+
+
+def test_batch_pre_pass_over_several_passes():
+    """70 channels at 61.44 Msps: several LDS passes, each reading its channels' pre-pass constants."""
+    from tests.test_gpu_signals import _channels
+    chans = []
+    for k, sig in enumerate(["L1CA", "E1", "L5", "B1I", "G1", "E5a", "B3I"]):
+        chans += _channels(sig, 10, 90 + k, 61_440)
+    sc = Scenario("many_batch", 61_440, "IQ4", 2, chans, noise=True, seed=33, gain_db=0.0)
+    print("case batch, 70 channels at 61.44 Msps from epoch 777", flush=True)
+    rows, masks = sc.window(777, 2)
+    _batch_vs_oracle(sc, rows, masks, first=777)
