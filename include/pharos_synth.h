@@ -103,6 +103,14 @@ const char *pharos_synth_last_error(void);
  * most 4096), built on the host with no device call, so it can be checked anywhere. */
 int pharos_synth_turn_table(double *out, uint32_t count);
 
+/* One AGC evaluation window on the host (Agc.account, gain.zig:19-35, as Run.pull applies it after each window,
+ * engine/run.zig:238-247): adds the window's clipped and total components, and at an evaluation epoch (epoch_after a
+ * multiple of 100) moves the gain by the clip rate.  A gain change is written to log[agc->adjustments] when that
+ * is below cap, and agc->adjustments counts it.  The same step the batched and graph forms run on the device.  No
+ * device call. */
+int pharos_synth_agc_account(pharos_synth_agc *agc, uint64_t clipped, uint64_t components, uint64_t epoch_after,
+                             pharos_synth_adjustment *log, uint32_t cap);
+
 /* Creates an engine on HIP device `device` (the tables are copied to it).  Replaces the table part of
  * Channels.create (src/products/run/engine/channels.zig:127-203) on the device side. */
 int pharos_synth_create(const pharos_synth_config *config, const pharos_synth_table *tables, uint32_t table_count,

@@ -95,6 +95,9 @@ def lib():
         "orc_rotor_values": (None, [C.c_double, C.c_double, C.c_uint32, C.c_uint32, VP, VP]),
         "orc_agc_account": (C.c_int, [P(Agc), C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_double)]),
         "orc_db_linear": (C.c_float, [C.c_double]),
+        "orc_symbol_locate": (C.c_int, [C.c_int, C.c_int, C.c_uint64, P(C.c_int64), P(C.c_uint32), P(C.c_uint32)]),
+        "orc_symbol_masks": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint32, VP, C.c_int64, C.c_uint32,
+                                       C.c_uint32, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -183,3 +186,26 @@ def synth_window(tables, rows, masks, spe, fmt, noise, seed, first_epoch, gain, 
     if want_sums:
         return out, clipped, sre, sim
     return out, clipped
+
+
+def symbol_locate(signal, prn, k):
+    """(frame, bit, nh chip) of code period k (SymbolSource.locate)."""
+    f, b, nh = C.c_int64(), C.c_uint32(), C.c_uint32()
+    if lib().orc_symbol_locate(SIGNAL[signal], prn, k, C.byref(f), C.byref(b), C.byref(nh)):
+        raise ValueError(signal)
+    return f.value, b.value, nh.value
+
+
+def symbol_masks(signal, prn, k0, n, frame_bits=None, frame0=0):
+    """Mask bytes of code periods k0 .. k0 + n - 1 (SymbolSource.fill); frame_bits [frames, bits] of 0/1 nav bits
+    from frame `frame0` (None: no frame has bits)."""
+    out = np.zeros(max(n, 1), dtype=np.uint8)
+    if frame_bits is None:
+        ptr, frames, bpf = None, 0, 0
+    else:
+        frame_bits = np.ascontiguousarray(frame_bits, dtype=np.uint8)
+        frames, bpf = frame_bits.shape
+        ptr = frame_bits.ctypes.data
+    if lib().orc_symbol_masks(SIGNAL[signal], prn, k0, n, ptr, frame0, frames, bpf, out.ctypes.data):
+        raise ValueError(signal)
+    return out[:n]

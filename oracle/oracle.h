@@ -140,6 +140,12 @@ int orc_source_values(const orc_table *t, const orc_row *p, const uint8_t *masks
 /* Carrier rotor values for samples [n0, n0+len) of an epoch (rotor.zig), as the kernel sees them. */
 void orc_rotor_values(double phase0, double step, uint32_t n0, uint32_t len, float *c_re, float *c_im);
 
+/* Symbol masks: src/products/run/symbols/source.zig:53-91.  locate maps code period k to its frame, bit and NH
+ * chip; masks fills n mask bytes from k0 with nav bits supplied per (frame, bit) by the caller. */
+int orc_symbol_locate(int signal, int prn, uint64_t k, int64_t *frame, uint32_t *bit, uint32_t *nh);
+int orc_symbol_masks(int signal, int prn, uint64_t k0, uint32_t n, const uint8_t *frame_bits, int64_t frame0,
+                     uint32_t frames, uint32_t bits_per_frame, uint8_t *out);
+
 /* AGC (gain.zig:14-94): feeds one window's counts; returns 1 when the gain changed (new gain in *gain). */
 typedef struct orc_agc { double gain; uint64_t clipped; uint64_t components; } orc_agc;
 int orc_agc_account(orc_agc *agc, uint64_t epoch_after, uint64_t clipped, uint64_t components, double *rate);

@@ -458,3 +458,58 @@ uint32_t orc__count(const orc_table *t) { return t->count; }
 uint32_t orc__epm(const orc_table *t) { return t->epm; }
 const double *orc__vre(const orc_table *t) { return t->vre; }
 const double *orc__vim(const orc_table *t) { return t->vim; }
+
+/* ---------------------------------------------------------------- symbol masks (symbols/source.zig) */
+/* catalog.frameFor (domain/signals/catalog.zig:74-77) with Spec.frameFor (model.zig:200-203): a BDS satellite
+ * that records/bds.zig:100-102 calls GEO (PRN <= 5 or >= 59) takes the row's GEO layout when it has one. */
+static void frame_for(int signal, int prn, uint32_t *code_ms, uint32_t *nh_len, uint32_t *nh_code, uint32_t *frame_ms) {
+    const spec *s = &SPECS[signal];
+    const int geo = s->system == ORC_BDS && (prn <= 5 || prn >= 59) && s->geo_frame_len != 0;
+    *code_ms = s->period_ms;
+    *nh_len = geo ? s->geo_nh_length : s->nh_length;
+    *nh_code = geo ? s->geo_nh_code : s->nh_code;
+    *frame_ms = geo ? s->geo_frame_len : s->frame_len;
+}
+
+/* SymbolSource.locate (source.zig:70-82). */
+int orc_symbol_locate(int signal, int prn, uint64_t k, int64_t *frame, uint32_t *bit, uint32_t *nh) {
+    if (!specs_ready) build_specs();
+    if (signal < 0 || signal >= ORC_SIGNALS) return -1;
+    uint32_t code_ms, nh_len, nh_code, frame_ms;
+    frame_for(signal, prn, &code_ms, &nh_len, &nh_code, &frame_ms);
+    const uint64_t bit_ms = (uint64_t)code_ms * nh_len;
+    const uint64_t ms = k * code_ms + SPECS[signal].frame_offset;
+    const uint64_t in_frame = ms % frame_ms;
+    const uint32_t nh_pos = (uint32_t)((in_frame % bit_ms) / code_ms);
+    *frame = (int64_t)(ms / frame_ms);
+    *bit = (uint32_t)(in_frame / bit_ms);
+    *nh = (nh_code >> nh_pos) & 1u;
+    return 0;
+}
+
+/* SymbolSource.fill (source.zig:53-59) with maskOf (:89-91) and pilotMask (:84-87): the mask of code period
+ * k0 + i is (navbit ^ nh) | secondary[k % len] << 1.  frame_bits[(f - frame0) * bits_per_frame + b] is bit b of
+ * frame f (0 or 1; anything else, or a frame outside [frame0, frame0 + frames), is a frame with no bits, read as
+ * 0, source.zig:56). */
+int orc_symbol_masks(int signal, int prn, uint64_t k0, uint32_t n, const uint8_t *frame_bits, int64_t frame0,
+                     uint32_t frames, uint32_t bits_per_frame, uint8_t *out) {
+    if (!specs_ready) build_specs();
+    if (signal < 0 || signal >= ORC_SIGNALS) return -1;
+    uint8_t sec[2048];
+    const int sl = orc_secondary_code(signal, prn, sec, (int)sizeof sec);
+    if (sl < 0) return -1;
+    for (uint32_t i = 0; i < n; i++) {
+        int64_t f;
+        uint32_t b, nh;
+        orc_symbol_locate(signal, prn, k0 + i, &f, &b, &nh);
+        uint32_t bit = 0;
+        if (frame_bits && f >= frame0 && f < frame0 + (int64_t)frames && b < bits_per_frame) {
+            const uint8_t v = frame_bits[(size_t)(f - frame0) * bits_per_frame + b];
+            bit = v == 1 ? 1u : 0u;
+        }
+        uint8_t m = (uint8_t)((bit ^ nh) * 1u);
+        if (sl > 0) m |= (uint8_t)(sec[(k0 + i) % (uint64_t)sl] * 2u);
+        out[i] = m;
+    }
+    return 0;
+}

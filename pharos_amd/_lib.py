@@ -66,7 +66,7 @@ class ChannelSpec(C.Structure):
 
 class ScenarioSpec(C.Structure):
     _fields_ = [("samples_per_epoch", C.c_uint32), ("channel_count", C.c_uint32), ("start_gps_ms", C.c_int64),
-                ("leap_seconds", C.c_int32), ("reserved", C.c_uint32)]
+                ("leap_seconds", C.c_int32), ("reserved", C.c_uint32), ("start_gps_sub_ms", C.c_double)]
 
 
 # pharos_synth_row as a numpy record (64 bytes, include/pharos_synth.h)
@@ -103,6 +103,7 @@ def lib():
         "pharos_synth_bytes_per_epoch": (U64, [VP]),
         "pharos_synth_device_bytes": (U64, [VP]),
         "pharos_synth_window": (C.c_int, [VP, U64, U32, U32, VP, VP, U64, F32, VP, P(U64), VP]),
+        "pharos_synth_agc_account": (C.c_int, [P(SynthAgc), U64, U64, U64, P(SynthAdjustment), U32]),
         "pharos_synth_batch": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP]),
         "pharos_synth_batch_submit": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP, P(VP)]),
         "pharos_synth_batch_query": (C.c_int, [VP]),
@@ -140,6 +141,7 @@ def lib():
         "pharos_scenario_mask_bound": (U64, [P(ScenarioSpec), P(ChannelSpec), U32]),
         "pharos_scenario_endpoint": (None, [P(ScenarioSpec), P(ChannelSpec), I64, P(C.c_double), P(I64), P(C.c_double)]),
         "pharos_derive": (C.c_int, [P(ScenarioSpec), P(ChannelSpec), U32, U64, VP]),
+        "pharos_scenario_nav_bit": (C.c_int, [P(ChannelSpec), I64, U32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

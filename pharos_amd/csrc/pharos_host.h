@@ -63,6 +63,7 @@ typedef struct pharos_scenario_spec {
     int64_t start_gps_ms;        // receiver time of epoch 0 (GPS ms)
     int32_t leap_seconds;        // GPS - UTC, for the GLONASS time scale
     uint32_t reserved;
+    double start_gps_sub_ms;     // sub-millisecond part of the start time, in [0, 1) (GpsTime.sub_ms)
 } pharos_scenario_spec;
 
 // Rows and masks of epochs [first_epoch, first_epoch + epochs): rows is [epochs][channel_count] and row c uses
@@ -77,6 +78,8 @@ uint64_t pharos_scenario_mask_bound(const pharos_scenario_spec *scenario, const 
 // One endpoint of a channel at receiver time t_rx_ms (exact ms): the carrier phase in cycles and the transmit time.
 void pharos_scenario_endpoint(const pharos_scenario_spec *scenario, const pharos_channel_spec *ch, int64_t t_rx_ms,
                               double *phase_cycles, int64_t *tx_ms, double *tx_sub_ms);
+// The stand-in's nav bit `bit` of frame `frame` of a channel (the message content SymbolSource reads; 0 or 1).
+int pharos_scenario_nav_bit(const pharos_channel_spec *ch, int64_t frame, uint32_t bit);
 // parameters.derive with the channel's constants; returns 1 for an active row.
 int pharos_derive(const pharos_scenario_spec *scenario, const pharos_channel_spec *ch, uint32_t table, uint64_t epoch,
                   pharos_synth_row *out);

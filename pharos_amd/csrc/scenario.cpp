@@ -63,6 +63,11 @@ Timing timing_of(const pharos_signal_constants &k, const pharos_channel_spec &c)
     return t;
 }
 
+// The stand-in's message content: one seeded random nav bit per (frame, bit) of a channel.
+uint8_t nav_bit(const pharos_channel_spec &c, uint64_t frame, uint64_t bit) {
+    return (uint8_t)(mix64(c.nav_seed ^ mix64(frame * 0x10001ull + bit)) & 1u);
+}
+
 // SymbolSource.fill (src/products/run/symbols/source.zig:53-89), with a random nav bit per (frame, bit).
 void fill_masks(const pharos_signal_constants &k, const pharos_channel_spec &c, uint64_t k0, uint64_t n, const uint8_t *secondary,
                 uint32_t sec_len, uint8_t *out) {
@@ -74,7 +79,7 @@ void fill_masks(const pharos_signal_constants &k, const pharos_channel_spec &c, 
         const uint64_t in_frame = ms % t.frame_ms;
         const uint32_t nh_pos = (uint32_t)((in_frame % bit_ms) / t.code_ms);
         const uint64_t frame = ms / t.frame_ms, bit = in_frame / bit_ms;
-        const uint8_t nav = (uint8_t)(mix64(c.nav_seed ^ mix64(frame * 0x10001ull + bit)) & 1u);
+        const uint8_t nav = nav_bit(c, frame, bit);
         const uint8_t nh = (uint8_t)((t.nh_code >> nh_pos) & 1u);
         uint8_t m = (uint8_t)(nav ^ nh);
         if (sec_len) m |= (uint8_t)(secondary[kk % sec_len] << 1);
@@ -96,6 +101,8 @@ struct Endpoint {
     double tx_sub;
 };
 
+// The receiver time is t_rx_ms + start_gps_sub_ms (every epoch edge shares the run's sub-millisecond part), so the
+// transmit time t_rx - d splits into whole ms t_rx_ms - ceil(d - sub) and the fraction ceil(d - sub) - (d - sub).
 Endpoint endpoint_at(const pharos_scenario_spec *s, const pharos_channel_spec *c, double lambda, int64_t t_rx_ms) {
     const double t = (double)(t_rx_ms - s->start_gps_ms) / 1000.0;
     double wobble = 0.0;
@@ -105,7 +112,7 @@ Endpoint endpoint_at(const pharos_scenario_spec *s, const pharos_channel_spec *c
     }
     const double cycles = -(c->doppler_hz * t + 0.5 * c->doppler_rate_hz_s * t * t) + wobble / lambda;
     const double range = c->range0_m + cycles * lambda;
-    const double d = range / LIGHT * 1000.0;  // travel time in ms
+    const double d = range / LIGHT * 1000.0 - s->start_gps_sub_ms;  // travel time in ms, less the sub-ms part
     const double up = ceil(d);
     return {cycles, t_rx_ms - (int64_t)up, up - d};
 }
@@ -129,7 +136,7 @@ int derive_from(const pharos_scenario_spec *s, const pharos_channel_spec *c, con
     const int64_t period0 = div_floor(bms, k.code_ms);
     const double in_period = (double)mod_floor(bms, k.code_ms) + sa;
     const double epm = (double)k.entries_per_ms;
-    const double phase0 = if_phase(c->if_hz, ta, 0.0) - pa;
+    const double phase0 = if_phase(c->if_hz, ta, s->start_gps_sub_ms) - pa;
     out->carrier_phase0 = phase0 - floor(phase0);
     out->carrier_step = (double)c->if_hz / (n * 1000.0) + (pa - pb) / n;
     out->entry_phase0 = in_period * epm;
@@ -165,6 +172,11 @@ int pharos_derive(const pharos_scenario_spec *s, const pharos_channel_spec *c, u
     const int64_t ta = s->start_gps_ms + (int64_t)epoch;
     return derive_from(s, c, k, table, epoch, endpoint_at(s, c, lambda, ta), endpoint_at(s, c, lambda, ta + 1),
                        (float)amplitude(c, s->samples_per_epoch), out);
+}
+
+int pharos_scenario_nav_bit(const pharos_channel_spec *c, int64_t frame, uint32_t bit) {
+    if (!c || frame < 0) return -2;
+    return nav_bit(*c, (uint64_t)frame, bit);
 }
 
 uint64_t pharos_scenario_mask_bound(const pharos_scenario_spec *s, const pharos_channel_spec *channels, uint32_t epochs) {

@@ -307,6 +307,37 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
 
 }  // namespace pharos
 
+// ------------------------------------------------------------------ host AGC step
+// Agc.account on the host (gain.zig:19-35), the same agc_step the device runs: for callers that run the plain form
+// and carry the gain between windows themselves, as Run.pull does (engine/run.zig:238-247).
+extern "C" int pharos_synth_agc_account(pharos_synth_agc *agc, uint64_t clipped, uint64_t components,
+                                        uint64_t epoch_after, pharos_synth_adjustment *log, uint32_t cap) {
+    g_error[0] = 0;
+    if (!agc || (cap && !log)) { set_error("null argument"); return ST_ARG; }
+    DevAgc G{};
+    G.gain = agc->gain;
+    G.clipped = agc->clipped;
+    G.components = agc->components;
+    G.adjustments = 0;
+    DevAdjust a{};
+    agc_step(G, clipped, components, epoch_after, &a, 1);
+    agc->gain = G.gain;
+    agc->clipped = G.clipped;
+    agc->components = G.components;
+    if (G.adjustments) {
+        if (agc->adjustments < cap) {
+            pharos_synth_adjustment &o = log[agc->adjustments];
+            o.epoch = a.epoch;
+            o.gain = a.gain;
+            o.clip_rate = a.rate;
+            o.reduced = a.reduced;
+            o.reserved = 0;
+        }
+        agc->adjustments++;
+    }
+    return ST_OK;
+}
+
 // ------------------------------------------------------------------ engine
 struct DeviceBuffer {
     void *p = nullptr;

@@ -141,6 +141,27 @@ class Engine:
         return np.frombuffer(clipped, dtype=np.uint64).copy() if clips else None
 
 
+class HostAgc:
+    """The adaptive gain carried between plain windows on the host, by the library's AGC step
+    (pharos_synth_agc_account: Agc.account, gain.zig:19-35, applied per window as engine/run.zig:238-247 does)."""
+
+    def __init__(self, cap=4096):
+        self.state = _lib.SynthAgc(1.0, 0, 0, 0, 0)
+        self.log = (_lib.SynthAdjustment * cap)()
+
+    @property
+    def gain(self):
+        return float(self.state.gain)
+
+    def account(self, clipped, components, epoch_after):
+        check(_lib.lib().pharos_synth_agc_account(C.byref(self.state), int(clipped), int(components),
+                                                  int(epoch_after), self.log, len(self.log)))
+
+    def adjustments(self):
+        n = min(self.state.adjustments, len(self.log))
+        return [(int(a.epoch), float(a.gain), float(a.clip_rate)) for a in self.log[:n]]
+
+
 def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=None, window=100, gain=None):
     """Per-window plain launches with the host AGC between windows (gain.zig:19-35, engine/run.zig:238-247).
 
@@ -154,31 +175,21 @@ def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=No
     constant = scenario.gain_db is not None or gain is not None
     g = gain if gain is not None else (float(np.float32(10.0 ** (scenario.gain_db / 20.0)))
                                        if scenario.gain_db is not None else 1.0)
-    agc_gain, acc_clip, acc_comp, adjustments = 1.0, 0, 0, []
+    agc = HostAgc()
     clips = np.zeros(epochs, dtype=np.uint64)
     e = 0
     while e < epochs:
         absolute = first_epoch + e
         n = min(window - absolute % window, epochs - e)
-        gw = g if constant else float(np.float32(agc_gain))
+        gw = g if constant else float(np.float32(agc.gain))
         c = eng.window(absolute, n, d_rows.at(e * nch * 64), d_masks.ptr, len(masks), gw, out.at(e * eng.bytes_per_epoch))
         clips[e:e + n] = c
         if not constant:
-            acc_clip += int(c.sum())
-            acc_comp += n * scenario.samples_per_epoch * 2
-            if (absolute + n) % 100 == 0:
-                rate = acc_clip / acc_comp
-                acc_clip = acc_comp = 0
-                if rate > 0.01:
-                    agc_gain *= 0.95
-                    adjustments.append((absolute + n, agc_gain, rate))
-                elif rate < 0.001 and agc_gain < 1.0:
-                    agc_gain = min(agc_gain * 1.02, 1.0)
-                    adjustments.append((absolute + n, agc_gain, rate))
+            agc.account(int(c.sum()), n * scenario.samples_per_epoch * 2, absolute + n)
         e += n
     result = out.to_numpy()
     eng.close()
-    return result, clips, agc_gain, adjustments
+    return result, clips, agc.gain, agc.adjustments()
 
 
 class Job:
@@ -236,29 +247,20 @@ def plain_job(job, stream=None, window=100):
     sc, eng = job.scenario, job.engine
     nch = len(sc.channels)
     cg = job.constant_gain
-    gain, acc_c, acc_n, total, adjustments = 1.0, 0, 0, 0, []
+    agc = HostAgc()
+    total = 0
     e = 0
     while e < job.epochs:
         absolute = job.first_epoch + e
         n = min(window - absolute % window, job.epochs - e)
-        gw = cg if cg is not None else float(np.float32(gain))
+        gw = cg if cg is not None else float(np.float32(agc.gain))
         c = eng.window(absolute, n, job.rows.at(e * nch * 64), job.masks.ptr, job.mask_len, gw,
                        job.out.at(e * eng.bytes_per_epoch), clips=True, stream=stream)
         total += int(c.sum())
         if cg is None:
-            acc_c += int(c.sum())
-            acc_n += n * sc.samples_per_epoch * 2
-            if (absolute + n) % 100 == 0:
-                rate = acc_c / acc_n
-                acc_c = acc_n = 0
-                if rate > 0.01:
-                    gain *= 0.95
-                    adjustments.append((absolute + n, gain, rate))
-                elif rate < 0.001 and gain < 1.0:
-                    gain = min(gain * 1.02, 1.0)
-                    adjustments.append((absolute + n, gain, rate))
+            agc.account(int(c.sum()), n * sc.samples_per_epoch * 2, absolute + n)
         e += n
-    return total, gain, adjustments
+    return total, agc.gain, agc.adjustments()
 
 
 def run_batch(jobs, workgroups_per_cu=0, stream=None, channels_per_pass=0):

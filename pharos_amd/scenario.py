@@ -51,6 +51,7 @@ class Scenario:
     seed: int = 0
     gain_db: Optional[float] = None  # None: the adaptive AGC
     start_gps_ms: int = DEFAULT_START_GPS_MS
+    start_gps_sub_ms: float = 0.0
     _tables: list = field(default=None, repr=False)
 
     @property
@@ -66,7 +67,8 @@ class Scenario:
         return len(self.channels) * self.samples_per_epoch * self.epochs
 
     def spec(self):
-        s = _lib.ScenarioSpec(self.samples_per_epoch, len(self.channels), self.start_gps_ms, LEAP_SECONDS, 0)
+        s = _lib.ScenarioSpec(self.samples_per_epoch, len(self.channels), self.start_gps_ms, LEAP_SECONDS, 0,
+                              self.start_gps_sub_ms)
         arr = (_lib.ChannelSpec * max(1, len(self.channels)))()
         for i, ch in enumerate(self.channels):
             arr[i] = ch.spec()
