@@ -94,7 +94,7 @@ def timed_batch(jobs, shape, stream):
     for j in jobs:
         _reset(j)
     arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
-    tuning = _lib.SynthTuning(shape[0], shape[1], (C.c_uint32 * 2)())
+    tuning = _lib.SynthTuning(shape[0], shape[1], 0, 0)
     e0, e1 = Event(), Event()
     h = C.c_void_p()
     L.pharos_synth_device_sync(0)

@@ -123,7 +123,8 @@ uint64_t pharos_synth_device_bytes(const pharos_synth_engine *engine);
 /* Plain form: one launch synthesises `epochs` epochs from first_epoch under one gain, like one window of
  * Run.runJob(.produce) (run.zig:367-380).  rows (device) is [epochs][channels]; masks (device) holds mask_len
  * bytes; out (device) receives epochs * bytes_per_epoch bytes.  clipped (host, optional) receives one count per
- * epoch.  Synchronises the stream when clipped is given. */
+ * epoch.  Synchronises the stream when clipped is given.  Windows of one engine on different streams run one after
+ * another on the device (each waits for the previous window's kernels), since they share the engine's clip counts. */
 int pharos_synth_window(pharos_synth_engine *engine, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
                         const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
                         uint8_t *out, uint64_t *clipped, void *stream);
@@ -154,8 +155,14 @@ typedef struct pharos_synth_job {
 typedef struct pharos_synth_tuning {
     uint32_t workgroups_per_cu;      /* persistent workgroups per compute unit */
     uint32_t channels_per_pass;      /* channels staged in LDS per pass of a tile, 1..32 */
-    uint32_t reserved[2];
+    uint32_t wait_limit;             /* polls (about a microsecond each) a tile waits for its window's gain before it
+                                        gives up; 0: 2^24.  A tile that gives up makes every later wait give up too,
+                                        the launch drains and the batch returns status 4. */
+    uint32_t flags;                  /* PHAROS_SYNTH_TUNE_* */
 } pharos_synth_tuning;
+/* Test of the wait bound: the host publishes no gain for each adaptive job's first window, so its tiles wait until
+ * the wait limit or pharos_synth_batch_abort ends the wait. */
+#define PHAROS_SYNTH_TUNE_HOLD_FIRST_GAIN 1u
 
 /* Runs all jobs to completion on `stream` and synchronises it.  Each job's bytes equal what the plain form
  * produces for it alone. */
@@ -168,6 +175,9 @@ typedef struct pharos_synth_batch_handle pharos_synth_batch_handle;
 int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t job_count, const pharos_synth_tuning *tuning,
                               void *stream, pharos_synth_batch_handle **out);
 int pharos_synth_batch_query(pharos_synth_batch_handle *handle); /* 1 done, 0 running, <0 error */
+/* Asks a submitted batch to stop waiting: tiles waiting for a window's gain give up (the launch drains with
+ * unusable output and pharos_synth_batch_wait returns status 4).  Status 2 when the handle has no abort word. */
+int pharos_synth_batch_abort(pharos_synth_batch_handle *handle);
 int pharos_synth_batch_wait(pharos_synth_batch_handle *handle);  /* frees the handle */
 
 /* Graph form: one job's windows as per-window launches with the AGC step between them on the device, captured once

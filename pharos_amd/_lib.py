@@ -44,7 +44,8 @@ class SynthJob(C.Structure):
 
 
 class SynthTuning(C.Structure):
-    _fields_ = [("workgroups_per_cu", C.c_uint32), ("channels_per_pass", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("workgroups_per_cu", C.c_uint32), ("channels_per_pass", C.c_uint32), ("wait_limit", C.c_uint32),
+                ("flags", C.c_uint32)]
 
 
 class SignalConstants(C.Structure):
@@ -108,6 +109,7 @@ def lib():
         "pharos_synth_batch_submit": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP, P(VP)]),
         "pharos_synth_batch_query": (C.c_int, [VP]),
         "pharos_synth_batch_wait": (C.c_int, [VP]),
+        "pharos_synth_batch_abort": (C.c_int, [VP]),
         "pharos_synth_to_file": (C.c_int, [P(SynthJob), C.c_char_p, P(U64)]),
         "pharos_synth_tag_text": (C.c_int, [U32, U64, I64, I64, C.c_char_p, C.c_size_t, P(C.c_size_t)]),
         "pharos_synth_graph_create": (C.c_int, [P(SynthJob), P(VP)]),

@@ -87,6 +87,8 @@ enum : uint32_t {
     ERR_MASKS = 2u,    // a row reads a mask beyond mask_count or the mask buffer
     ERR_TABLE = 4u,    // a row names a table out of range or a zero code rate
     ERR_LAYOUT = 8u,   // the build placed the value-table slots away from LDS address 0
+    ERR_WAIT = 16u,    // a tile stopped waiting for its window's gain after the wait limit (batched form)
+    ERR_ABORT = 32u,   // the host aborted the batch while tiles waited for a gain (batched form)
 };
 
 // Work description for the persistent batched kernel: one segment = one window of one job.

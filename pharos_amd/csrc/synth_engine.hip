@@ -105,6 +105,9 @@ struct BatchState {
     uint32_t queue_words_offset;   // LDS byte offset of the work-queue words
     uint32_t chunk_cap;            // channels per LDS pass at most (0: each engine's own)
     const uint32_t *seg_of_block;  // the segment holding item k * SEG_BLOCK, for k <= total items / SEG_BLOCK
+    uint32_t wait_limit;           // polls of a gain word before a tile gives up
+    uint32_t *gave_up;             // set by the first tile that gives up; later waits end at once
+    const uint32_t *abort_word;    // host-written (pinned, coherent), or null
 };
 constexpr uint32_t SEG_BLOCK = 256;
 
@@ -223,6 +226,30 @@ __global__ __launch_bounds__(256) void row_prep_kernel(const pharos_synth_row *r
     out[i] = p;
 }
 
+// The gain a tile of segment s quantises with, once the previous window's AGC step has published it.  The wait is
+// bounded: it gives up after wait_limit polls, when another tile has given up, or when the host sets the batch's
+// abort word; then it raises ERR_WAIT or ERR_ABORT and the tile goes on with the gain word as it stands (1.0 if
+// none was published), so every workgroup drains and the launch ends.
+__device__ uint32_t wait_gain(const BatchState &S, uint32_t s) {
+    unsigned long long w = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t polls = 0; !(w & READY_BIT); polls++) {
+        uint32_t why = 0;
+        if (__hip_atomic_load(S.gave_up, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) why = ERR_WAIT;
+        else if (polls >= S.wait_limit) why = ERR_WAIT;
+        else if (S.abort_word && (polls & 63u) == 0 &&
+                 __hip_atomic_load(S.abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+            why = ERR_ABORT;
+        if (why) {
+            atomicOr(S.err, why);
+            __hip_atomic_store(S.gave_up, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return __float_as_uint(1.0f);
+        }
+        __builtin_amdgcn_s_sleep(8);
+        w = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return (uint32_t)w;
+}
+
 template <uint32_t F>
 __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -273,12 +300,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
         A.prep = J.prep ? J.prep + (size_t)e_off * J.channels : nullptr;
         auto gain_fn = [&]() -> float {
             if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT) return J.constant_gain;
-            if (threadIdx.x == 0) {
-                unsigned long long w;
-                while (!((w = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) & READY_BIT))
-                    __builtin_amdgcn_s_sleep(8);
-                q[3] = (uint32_t)w;
-            }
+            if (threadIdx.x == 0) q[3] = wait_gain(S, s);
             __syncthreads();
             return __uint_as_float(q[3]);
         };
@@ -355,6 +377,9 @@ struct pharos_synth_engine {
     uint32_t *d_clip = nullptr;
     size_t clip_cap = 0;
     std::vector<uint32_t> h_clip;
+    // Recorded after each plain window's launches: the next window, on whatever stream, waits for it before its
+    // kernels write d_clip again, so windows of one engine on two streams never overwrite each other's counts.
+    hipEvent_t clip_free = nullptr;
     std::mutex lock;
 };
 
@@ -375,6 +400,10 @@ static int dev_alloc(pharos_synth_engine *e, void **p, size_t n) {
 static void engine_free(pharos_synth_engine *e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
+    if (e->clip_free) {
+        (void)hipEventSynchronize(e->clip_free);
+        (void)hipEventDestroy(e->clip_free);
+    }
     for (void *p : e->owned) (void)hipFree(p);
     delete e;
 }
@@ -660,6 +689,12 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
         engine_free(e);
         return ST_INTERNAL;
     }
+    if (hipEventCreateWithFlags(&e->clip_free, hipEventDisableTiming) != hipSuccess) {
+        e->clip_free = nullptr;
+        set_error("event creation failed");
+        engine_free(e);
+        return ST_INTERNAL;
+    }
     e->d_engine = (DevEngine *)(base + engine_off);
     e->d_err = (uint32_t *)(base + err_off);
     *out = e;
@@ -685,33 +720,18 @@ static int take_device_error(pharos_synth_engine *e, hipStream_t stream) {
     return ST_INTERNAL;
 }
 
-extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
-                                   const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
-                                   uint8_t *out, uint64_t *clipped, void *stream_) {
-    g_error[0] = 0;
-    if (!e || !out || (channels && !rows)) { set_error("null argument"); return ST_ARG; }
-    if (epochs == 0) { set_error("epochs must be at least 1"); return ST_ARG; }
-    if (!(gain > 0.0f) || !isfinite(gain)) { set_error("gain must be positive and finite"); return ST_ARG; }
-    std::lock_guard<std::mutex> guard(e->lock);
-    hipStream_t stream = (hipStream_t)stream_;
-    HIP_TRY(hipSetDevice(e->device), ST_INTERNAL);
+// The plain form's launches: grid (tiles, epochs) in pieces of at most 65535 epochs; clip_partial receives one
+// count per (epoch, tile).
+static int launch_window_tiles(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+                               const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
+                               uint8_t *out, uint32_t *clip_partial, hipStream_t stream) {
     const uint32_t tiles = e->view.tiles;
-    const size_t nclip = (size_t)epochs * tiles;
-    if (nclip > e->clip_cap) {
-        if (e->d_clip) {
-            HIP_TRY(hipStreamSynchronize(stream), ST_INTERNAL);
-            (void)hipFree(e->d_clip);
-            e->d_clip = nullptr;
-        }
-        HIP_TRY(hipMalloc((void **)&e->d_clip, nclip * sizeof(uint32_t)), ST_NOMEM);
-        e->clip_cap = nclip;
-    }
     for (uint32_t done = 0; done < epochs;) {
         const uint32_t n = std::min<uint32_t>(epochs - done, 65535u);
         const dim3 grid(tiles, n);
         const pharos_synth_row *r = rows + (size_t)done * channels;
         uint8_t *o = out + (size_t)done * e->view.bpe;
-        uint32_t *cp = e->d_clip + (size_t)done * tiles;
+        uint32_t *cp = clip_partial + (size_t)done * tiles;
         const uint32_t lds = e->view.lds_bytes;
         switch (e->view.format) {
 #define PHAROS_WINDOW(FMT)                                                                                         \
@@ -729,17 +749,47 @@ extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch,
         HIP_TRY(hipGetLastError(), ST_INTERNAL);
         done += n;
     }
+    return ST_OK;
+}
+
+extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+                                   const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
+                                   uint8_t *out, uint64_t *clipped, void *stream_) {
+    g_error[0] = 0;
+    if (!e || !out || (channels && !rows)) { set_error("null argument"); return ST_ARG; }
+    if (epochs == 0) { set_error("epochs must be at least 1"); return ST_ARG; }
+    if (!(gain > 0.0f) || !isfinite(gain)) { set_error("gain must be positive and finite"); return ST_ARG; }
+    std::lock_guard<std::mutex> guard(e->lock);
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(e->device), ST_INTERNAL);
+    const uint32_t tiles = e->view.tiles;
+    const size_t nclip = (size_t)epochs * tiles;
+    if (nclip > e->clip_cap) {
+        if (e->d_clip) {
+            HIP_TRY(hipEventSynchronize(e->clip_free), ST_INTERNAL);
+            (void)hipFree(e->d_clip);
+            e->d_clip = nullptr;
+        }
+        HIP_TRY(hipMalloc((void **)&e->d_clip, nclip * sizeof(uint32_t)), ST_NOMEM);
+        e->clip_cap = nclip;
+    }
+    HIP_TRY(hipStreamWaitEvent(stream, e->clip_free, 0), ST_INTERNAL);
+    int st = launch_window_tiles(e, first_epoch, epochs, channels, rows, masks, mask_len, gain, out, e->d_clip, stream);
+    if (st) return st;
     if (clipped) {
         e->h_clip.resize(nclip);
         HIP_TRY(hipMemcpyAsync(e->h_clip.data(), e->d_clip, nclip * sizeof(uint32_t), hipMemcpyDeviceToHost, stream), ST_INTERNAL);
-        int st = take_device_error(e, stream);
+        HIP_TRY(hipEventRecord(e->clip_free, stream), ST_INTERNAL);
+        st = take_device_error(e, stream);
         if (st) return st;
         for (uint32_t k = 0; k < epochs; k++) {
             uint64_t s = 0;
             for (uint32_t t = 0; t < tiles; t++) s += e->h_clip[(size_t)k * tiles + t];
             clipped[k] = s;
         }
+        return ST_OK;
     }
+    HIP_TRY(hipEventRecord(e->clip_free, stream), ST_INTERNAL);
     return ST_OK;
 }
 
@@ -761,7 +811,43 @@ struct pharos_synth_batch_handle {
     // counts, error flags) copied back into it after the launch, both on the batch's stream.
     uint8_t *pinned = nullptr;
     size_t pinned_bytes = 0, at_agc = 0, at_count = 0, at_err = 0;
+    int abort_slot = -1;           // this batch's word of the shared abort array, or -1
 };
+
+// Abort words of running batches: one pinned, coherent host array shared by the process, one word per batch in
+// flight, which the device polls while a tile waits for a gain.
+constexpr int ABORT_SLOTS = 1024;
+static uint32_t *g_abort_host = nullptr;
+static uint32_t *g_abort_dev = nullptr;
+static bool g_abort_used[ABORT_SLOTS];
+static std::mutex g_abort_mu;
+
+static int abort_take() {
+    std::lock_guard<std::mutex> guard(g_abort_mu);
+    if (!g_abort_host) {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, ABORT_SLOTS * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+            return -1;
+        void *d = nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) { (void)hipHostFree(p); return -1; }
+        g_abort_host = (uint32_t *)p;
+        g_abort_dev = (uint32_t *)d;
+        memset(p, 0, ABORT_SLOTS * sizeof(uint32_t));
+    }
+    for (int i = 0; i < ABORT_SLOTS; i++)
+        if (!g_abort_used[i]) {
+            g_abort_used[i] = true;
+            __atomic_store_n(&g_abort_host[i], 0u, __ATOMIC_RELEASE);
+            return i;
+        }
+    return -1;
+}
+
+static void abort_release(int slot) {
+    if (slot < 0) return;
+    std::lock_guard<std::mutex> guard(g_abort_mu);
+    g_abort_used[slot] = false;
+}
 
 // Batch-state blocks outlive their batch in a small per-process cache: a serving loop submits a batch per
 // iteration, and a device allocation and free per batch would cost more than its tables.  A block goes back to the
@@ -798,6 +884,7 @@ static void batch_free(pharos_synth_batch_handle *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->done) (void)hipEventDestroy(h->done);
+    abort_release(h->abort_slot);
     for (size_t i = 0; i < h->owned.size(); i++) {
         std::lock_guard<std::mutex> guard(g_spare_lock);
         if (g_spare.size() < SPARE_MAX) g_spare.push_back({h->device, h->owned[i], h->owned_bytes[i]});
@@ -923,6 +1010,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     for (uint32_t j = 0; j < njobs; j++) adjust_cap = std::max(adjust_cap, jobs[j].adjustments_cap);
     h->adjust_cap = adjust_cap;
 
+    const bool hold_first = tuning && (tuning->flags & PHAROS_SYNTH_TUNE_HOLD_FIRST_GAIN);
     std::vector<DevJob> dj(njobs);
     std::vector<DevAgc> agc(njobs);
     std::vector<float> seg_gain(nsegs, 1.0f);
@@ -949,7 +1037,7 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
         }
         const uint32_t f = h->job_first_seg[j];
         seg_gain[f] = J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE ? (float)J.agc->gain : J.constant_gain;
-        seg_ready[f] = 1;
+        seg_ready[f] = hold_first && J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE ? 0 : 1;
     }
     // Device buffers: one allocation carved into the pieces.  The tables the kernel reads come first, then the
     // result words (copied back after the launch), then the adjustment logs (written by the device only, read back
@@ -1025,7 +1113,11 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     S.adjust = (DevAdjust *)(b8 + p_adj.off);
     S.adjust_cap = adjust_cap;
     S.err = (uint32_t *)(b8 + p_err.off);
+    S.gave_up = (uint32_t *)(b8 + p_err.off + 4);
     S.seg_of_block = (const uint32_t *)(b8 + p_sob.off);
+    S.wait_limit = tuning && tuning->wait_limit ? tuning->wait_limit : (1u << 24);
+    h->abort_slot = abort_take();
+    S.abort_word = h->abort_slot >= 0 ? g_abort_dev + h->abort_slot : nullptr;
     S.queue_words_offset = (batch_lds + 15u) & ~15u;
     S.chunk_cap = tuning ? std::min<uint32_t>(tuning->channels_per_pass, MAX_CHUNK) : 0;
     h->d_agc = S.agc;
@@ -1100,6 +1192,14 @@ extern "C" int pharos_synth_batch_query(pharos_synth_batch_handle *h) {
     return -ST_INTERNAL;
 }
 
+extern "C" int pharos_synth_batch_abort(pharos_synth_batch_handle *h) {
+    g_error[0] = 0;
+    if (!h) { set_error("null handle"); return ST_ARG; }
+    if (h->abort_slot < 0) { set_error("this batch has no abort word"); return ST_ARG; }
+    __atomic_store_n(&g_abort_host[h->abort_slot], 1u, __ATOMIC_RELEASE);
+    return ST_OK;
+}
+
 extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
     if (!h) { set_error("null handle"); return ST_ARG; }
     (void)hipSetDevice(h->device);
@@ -1147,7 +1247,11 @@ extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
             }
         }
     }
-    if (err) {
+    if (err & (ERR_WAIT | ERR_ABORT)) {
+        set_error("%s: tiles stopped waiting for a window's gain, the output is not usable (device flags 0x%x)",
+                  (err & ERR_ABORT) ? "batch aborted by the host" : "gain wait limit reached", err);
+        status = ST_INTERNAL;
+    } else if (err) {
         set_error("signal parameters are outside the supported calculation range (device flags 0x%x)", err);
         status = ST_INTERNAL;
     }

@@ -263,12 +263,39 @@ def plain_job(job, stream=None, window=100):
     return total, agc.gain, agc.adjustments()
 
 
-def run_batch(jobs, workgroups_per_cu=0, stream=None, channels_per_pass=0):
+def run_batch(jobs, workgroups_per_cu=0, stream=None, channels_per_pass=0, wait_limit=0, flags=0):
     """Runs jobs (a list of Job) in one persistent launch and waits for it."""
     arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
-    tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, (C.c_uint32 * 2)())
+    tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, wait_limit, flags)
     check(_lib.lib().pharos_synth_batch(arr, len(jobs), C.byref(tuning), stream.handle if stream else None))
     return jobs
+
+
+class Batch:
+    """A submitted batch (pharos_synth_batch_submit): query, abort and wait."""
+
+    def __init__(self, jobs, workgroups_per_cu=0, channels_per_pass=0, wait_limit=0, flags=0, stream=None):
+        self.jobs = jobs
+        self._arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
+        tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, wait_limit, flags)
+        h = C.c_void_p()
+        check(_lib.lib().pharos_synth_batch_submit(self._arr, len(jobs), C.byref(tuning),
+                                                   stream.handle if stream else None, C.byref(h)))
+        self.handle = h.value
+
+    def done(self):
+        st = _lib.lib().pharos_synth_batch_query(self.handle)
+        if st < 0:
+            raise _lib.PharosError(-st, _lib.last_error())
+        return st == 1
+
+    def abort(self):
+        check(_lib.lib().pharos_synth_batch_abort(self.handle))
+
+    def wait(self):
+        h, self.handle = self.handle, None
+        check(_lib.lib().pharos_synth_batch_wait(h))
+        return self.jobs
 
 
 class Graph:

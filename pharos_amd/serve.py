@@ -224,7 +224,7 @@ class Server:
         require_gpu()
         self.device = device
         self.reserve = reserve
-        self.tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, (C.c_uint32 * 2)())
+        self.tuning = _lib.SynthTuning(workgroups_per_cu, channels_per_pass, 0, 0)
         threads = threads or min(16, os.cpu_count() or 4)
         # Two pools: the next slices of admitted requests, which the next batch waits for, never queue behind the
         # work of requests that just arrived (their engines and first slices).

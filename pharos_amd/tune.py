@@ -43,7 +43,7 @@ def _time(jobs, w, c, stream, repeats):
         for j in jobs:
             j.agc.gain, j.agc.clipped, j.agc.components, j.agc.adjustments = 1.0, 0, 0, 0
         arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
-        tuning = _lib.SynthTuning(w, c, (C.c_uint32 * 2)())
+        tuning = _lib.SynthTuning(w, c, 0, 0)
         e0, e1 = Event(), Event()
         h = C.c_void_p()
         e0.record(stream)
