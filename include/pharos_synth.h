@@ -129,6 +129,16 @@ int pharos_synth_window(pharos_synth_engine *engine, uint64_t first_epoch, uint3
                         const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
                         uint8_t *out, uint64_t *clipped, void *stream);
 
+/* The plain form without any allocation, for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid allocation).  No counterpart in the reference.  Same arguments, kernels and
+ * results as pharos_synth_window; the per-tile clip counts and per-epoch totals live in a static device array of
+ * 3 MiB per device (held until the process ends) instead of engine memory, so epochs x tiles per call is at most
+ * 262144 (tiles = ceil(samples_per_epoch / 4096)); more is status 2.  Calls that overlap on one device share that
+ * array and race: a caller runs them one at a time per device (each call with `clipped` returns synchronised). */
+int pharos_synth_window_static(pharos_synth_engine *engine, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+                               const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
+                               uint8_t *out, uint64_t *clipped, void *stream);
+
 /* Batched form: many runs (jobs) of any engines, rates and formats in one persistent launch over a work queue of
  * (job, window, tile) items.  Windows are the AGC evaluation windows (epoch multiples of 100, gain.zig:5) and the
  * adaptive gain is carried on the device from window to window, so no host round trip separates windows. */

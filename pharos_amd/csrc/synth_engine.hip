@@ -57,6 +57,16 @@ __global__ void clear_words(uint32_t *p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
+// Per-epoch clip totals from per-tile counts (one thread per epoch).
+__global__ __launch_bounds__(256) void epoch_clip_sums(const uint32_t *partial, uint32_t tiles, uint32_t epochs,
+                                                       unsigned long long *sums) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= epochs) return;
+    unsigned long long s = 0;
+    for (uint32_t t = 0; t < tiles; t++) s += partial[(size_t)e * tiles + t];
+    sums[e] = s;
+}
+
 // One kernel per output format: each carries only its own quantiser, which keeps the code within the instruction
 // cache.
 template <uint32_t F>
@@ -542,6 +552,31 @@ static int shared_turn_table(int device, const double **out) {
     return ST_OK;
 }
 
+// Scratch of the window form that allocates nothing (pharos_synth_window_static): per-tile clip counts, then
+// per-epoch totals.  One copy per device, for the life of the process; its device address is taken once per device
+// at engine creation (the host-side name of a __device__ array is not a device address).
+constexpr uint32_t STATIC_TILES = 1u << 18;  // epochs x tiles of one call, at most
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)STATIC_TILES * sizeof(uint32_t) + (size_t)STATIC_TILES * sizeof(unsigned long long);
+static __device__ __align__(256) uint8_t scratch[STATIC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch[256];
+
+// With `device` current: the device address of its copy of the scratch array, taken on first use.
+static int static_scratch(int device, uint8_t **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (st != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(st));
+            return ST_INTERNAL;
+        }
+        g_scratch[device] = (uint8_t *)p;
+    }
+    *out = g_scratch[device];
+    return ST_OK;
+}
+
 extern "C" int pharos_synth_create(const pharos_synth_config *config, const pharos_synth_table *tables, uint32_t table_count,
                                    int device, pharos_synth_engine **out) {
     g_error[0] = 0;
@@ -640,6 +675,8 @@ extern "C" int pharos_synth_create(const pharos_synth_config *config, const phar
     const double *turn = nullptr;
     int st;
     if ((st = shared_turn_table(device, &turn))) { engine_free(e); return st; }
+    uint8_t *scratch_ptr = nullptr;
+    if ((st = static_scratch(device, &scratch_ptr))) { engine_free(e); return st; }
     void *p;
     if ((st = dev_alloc(e, &p, total))) { engine_free(e); return st; }
     uint8_t *base = (uint8_t *)p;
@@ -790,6 +827,39 @@ extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch,
         return ST_OK;
     }
     HIP_TRY(hipEventRecord(e->clip_free, stream), ST_INTERNAL);
+    return ST_OK;
+}
+
+// The plain form with no allocation: the same kernels, with the per-tile clip counts and per-epoch totals in the
+// device's static scratch array instead of memory the engine allocates.  Calls on one device share that array.
+extern "C" int pharos_synth_window_static(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs,
+                                          uint32_t channels, const pharos_synth_row *rows, const uint8_t *masks,
+                                          uint64_t mask_len, float gain, uint8_t *out, uint64_t *clipped,
+                                          void *stream_) {
+    g_error[0] = 0;
+    if (!e || !out || (channels && !rows)) { set_error("null argument"); return ST_ARG; }
+    if (epochs == 0) { set_error("epochs must be at least 1"); return ST_ARG; }
+    if (!(gain > 0.0f) || !isfinite(gain)) { set_error("gain must be positive and finite"); return ST_ARG; }
+    const uint32_t tiles = e->view.tiles;
+    if ((uint64_t)epochs * tiles > STATIC_TILES) {
+        set_error("%u epochs of %u tiles exceed the static scratch's %u tiles per call (at most %u epochs)", epochs,
+                  tiles, STATIC_TILES, STATIC_TILES / tiles);
+        return ST_ARG;
+    }
+    uint8_t *base = g_scratch[e->device];  // taken when the engine was created on this device
+    if (!base) { set_error("internal: no scratch address for device %d", e->device); return ST_INTERNAL; }
+    uint32_t *partial = (uint32_t *)base;
+    unsigned long long *sums = (unsigned long long *)(base + (size_t)STATIC_TILES * sizeof(uint32_t));
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(e->device), ST_INTERNAL);
+    int st = launch_window_tiles(e, first_epoch, epochs, channels, rows, masks, mask_len, gain, out, partial, stream);
+    if (st) return st;
+    if (clipped) {
+        hipLaunchKernelGGL(epoch_clip_sums, dim3((epochs + 255) / 256), dim3(256), 0, stream, partial, tiles, epochs, sums);
+        HIP_TRY(hipGetLastError(), ST_INTERNAL);
+        HIP_TRY(hipMemcpyAsync(clipped, sums, (size_t)epochs * sizeof(uint64_t), hipMemcpyDeviceToHost, stream), ST_INTERNAL);
+        return take_device_error(e, stream);
+    }
     return ST_OK;
 }
 
