@@ -8,7 +8,9 @@ adaptive AGC -- rows and masks resident in HBM before the timed region, the 3.69
   graph        the same run as per-window launches with the AGC step on the device, captured once as a HIP graph
   plain        the same run as per-window plain launches with the host AGC in between
   roofline     FP32 VALU roofline of the synthesis kernel: algorithmic flops per launch / launch time
-  cpu_baseline the C oracle (a restatement of the reference kernel) on the host cores, on a bounded sample
+  cpu_baseline the C oracle (a restatement of the reference kernel), built for this host, on all its cores and on
+               one thread, on a bounded sample
+  precision    the f32 carrier rotor against FP64 and the Q32.32 code position against f64, every sample of the run
   run_latency  p50/p99 wall time of the timed batched runs
   serving      p50/p99 latency of a Poisson request stream (100 req/s for 10 s by default; --no-serve skips it)
 
@@ -33,6 +35,7 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic FP32 flops per channel-sample (kernel.zig:241-258 plus rotor.zig:52-58): a complex MAC and a rotor
 # step for two-axis signals (8 + 6), two FMAs and a rotor step for one-axis signals (4 + 6).
 FLOPS_TWO_AXIS, FLOPS_ONE_AXIS = 14, 10
+CPU_SHARE = 16  # host cores one GPU's job may use on a GPU box
 
 
 def flops_per_epoch(scenario):
@@ -47,23 +50,95 @@ def flops_per_epoch(scenario):
     return total * scenario.samples_per_epoch
 
 
-def cpu_baseline(scenario, seconds=10.0, threads=None):
-    """The oracle over 100-epoch windows of the same workload until `seconds` of wall time have passed."""
-    from oracle import binding as O
-    threads = threads or min(16, os.cpu_count() or 1)
-    tabs = O.Tables([(c.signal, c.prn) for c in scenario.channels])
+def host_cpu():
+    """The host's CPUs as this process sees them: nproc, the cores its affinity mask allows, the cgroup's CPU quota
+    (cores, None when unlimited) and the processor model."""
+    info = {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)), "cgroup_quota_cores": None,
+            "model": None}
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            info["cgroup_quota_cores"] = int(quota) / int(period)
+    except (OSError, ValueError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return info
+
+
+def _oracle_rate(O, tabs, scenario, seconds, threads, max_epochs):
+    """Channel-samples/s of the oracle over consecutive 100-epoch windows of the run, for `seconds` of wall time."""
     nch = len(scenario.channels)
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds and done < scenario.epochs:
-        rows, masks = scenario.window(done, 100)
-        O.synth_window(tabs, rows, masks, scenario.samples_per_epoch, scenario.format_id, 0 if scenario.noise else 1,
-                       scenario.seed, done, 1.0, threads=threads)
+    done, busy, t0 = 0, 0.0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds and done < max_epochs:
+        rows, masks = scenario.window(done, 100)   # the host stage, outside the timed part
+        t1 = time.perf_counter()
+        O.synth_window(tabs, rows, masks, scenario.samples_per_epoch, scenario.format_id,
+                       0 if scenario.noise else 1, scenario.seed, done, 1.0, threads=threads)
+        busy += time.perf_counter() - t1
         done += 100
-    dt = time.perf_counter() - t0
-    return {"value": nch * scenario.samples_per_epoch * done / dt, "unit": "channel-samples/s", "cores": threads,
-            "kind": "port",
-            "sample": f"first {done} epochs ({done / 1000:.1f} s of signal) of the same {nch}-channel run, "
-                      f"C restatement of kernel.zig on {threads} threads, constant gain, {dt:.1f} s wall"}
+    return nch * scenario.samples_per_epoch * done / busy, done, busy
+
+
+def cpu_baseline(scenario, seconds=10.0, single_seconds=4.0):
+    """The oracle (the C restatement of the reference kernel), built -O3 -march=native for this host, on every core
+    this process may use, over 100-epoch windows of the same workload; then on one thread.  Windows are timed
+    around the synthesis alone (the reference's derive runs on its caller thread, not in the pool)."""
+    import tempfile
+    from oracle import binding as O
+    host = host_cpu()
+    threads = host["affinity_cores"]
+    if host["cgroup_quota_cores"]:
+        threads = max(1, min(threads, int(host["cgroup_quota_cores"] + 0.5)))
+    # A GPU box grants one GPU's job a share of 16 host cores, whatever nproc shows; the pool stays within it.
+    threads = min(threads, CPU_SHARE)
+    build = "-O3 -march=native -ffp-contract=off"
+    try:
+        O.use(O.build_native(tempfile.mkdtemp(prefix="pharos_oracle_")))
+    except Exception as e:  # no compiler on the host: the prebuilt x86-64-v3 library
+        build = f"-O3 -march=x86-64-v3 (native build failed: {e})"
+    tabs = O.Tables([(c.signal, c.prn) for c in scenario.channels])
+    all_rate, all_epochs, all_busy = _oracle_rate(O, tabs, scenario, seconds, threads, scenario.epochs)
+    one_rate, one_epochs, one_busy = _oracle_rate(O, tabs, scenario, single_seconds, 1, scenario.epochs)
+    nch = len(scenario.channels)
+    return {"value": all_rate, "unit": "channel-samples/s", "cores": threads, "kind": "port",
+            "sample": f"first {all_epochs} epochs ({all_epochs / 1000:.1f} s of signal) of the same {nch}-channel run, "
+                      f"C restatement of kernel.zig ({build}) on {threads} threads, constant gain, "
+                      f"{all_busy:.1f} s of synthesis",
+            "single_thread": {"value": one_rate, "epochs": one_epochs, "seconds": one_busy},
+            "parallel_speedup": all_rate / one_rate, "build": build, "host": host,
+            # Every core of the machine at the single-thread rate, an upper bound of an all-core run that the job's
+            # CPU share cannot measure.
+            "all_nproc_linear_bound": one_rate * host["nproc"]}
+
+
+def precision(job):
+    """pharos_synth_precision over every row of the job: largest carrier phase error of the f32 rotor against FP64
+    and largest Q32.32 code position error, over the whole run."""
+    import ctypes as C
+    from pharos_amd import _lib
+    sc = job.scenario
+    nch = len(sc.channels)
+    rep = _lib.PrecisionReport()
+    per_table = (C.c_double * max(1, nch))()
+    t0 = time.perf_counter()
+    _lib.check(_lib.lib().pharos_synth_precision(job.engine.handle, job.rows.ptr, job.epochs * nch, C.byref(rep),
+                                                 per_table, None))
+    chips = 0.0
+    for c, ch in enumerate(sc.channels):
+        k = _lib.SignalConstants()
+        _lib.lib().pharos_signal_info(_lib.SIGNAL[ch.signal], C.byref(k))
+        chips = max(chips, per_table[c] / k.entries_per_chip)
+    return {"carrier_max_err_cycles": rep.carrier_cycles, "carrier_max_err_rad": rep.carrier_radians,
+            "rotor_magnitude_max_err": rep.magnitude, "code_max_err_entries": rep.code_entries,
+            "code_max_err_chips": chips, "rows": rep.rows, "samples": job.epochs * nch * sc.samples_per_epoch,
+            "bound_1e6_cycles_met": rep.carrier_cycles <= 1e-6, "seconds": time.perf_counter() - t0,
+            "against": "FP64 phasor of phase0 + n*step and f64 entry_phase0 + n*eps, every sample of every row"}
 
 
 def profiled(epochs):
@@ -228,9 +303,15 @@ def bench_single(args):
         "tuning": tuned,
         "run_latency": run_latency(runs),
     }
+    if not args.no_precision:
+        line["precision"] = precision(job)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc, args.cpu_seconds)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+        line["gpu_over_cpu_single_thread"] = value / line["cpu_baseline"]["single_thread"]["value"]
+        line["gpu_over_cpu_nproc_bound"] = value / line["cpu_baseline"]["all_nproc_linear_bound"]
+        line["target_50x_all_core_met"] = line["gpu_over_cpu"] >= 50.0
+        line["target_50x_nproc_bound_met"] = line["gpu_over_cpu_nproc_bound"] >= 50.0
     return line
 
 
@@ -318,6 +399,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true", help="skip the whole-run precision report")
     ap.add_argument("--serve", action="store_true", help="run the Poisson serving measurement (the default)")
     ap.add_argument("--no-serve", action="store_true", help="skip the Poisson serving measurement")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second")

@@ -218,6 +218,22 @@ int pharos_synth_to_file(const pharos_synth_job *job, const char *path, uint64_t
 int pharos_synth_tag_text(uint32_t format, uint64_t sample_rate_hz, int64_t center_hz, int64_t first_gps_ms,
                           char *buf, size_t cap, size_t *len);
 
+/* Precision of the split-precision path over a run's rows (a diagnostic, outside any timed path): for every sample
+ * of every active row, the f32 carrier rotor the kernels run (rotor.zig:12-58: f64 seeds every 512 samples, f32
+ * steps between) against the FP64 phasor of phase0 + n * step, and the Q32.32 code position (chip.zig:12-29)
+ * against the f64 entry_phase0 + n * entries_per_sample.  rows is device memory; code_error_per_table (host,
+ * optional) receives the largest code error of each of the engine's tables, in entries. */
+typedef struct pharos_synth_precision_report {
+    double carrier_cycles;   /* largest carrier phase error, cycles */
+    double carrier_radians;  /* the same in radians */
+    double magnitude;        /* largest | |rotor| - 1 | */
+    double code_entries;     /* largest code position error, code entries (entries_per_chip per chip) */
+    uint64_t rows;
+    uint64_t reserved;
+} pharos_synth_precision_report;
+int pharos_synth_precision(pharos_synth_engine *engine, const pharos_synth_row *rows, uint64_t nrows,
+                           pharos_synth_precision_report *report, double *code_error_per_table, void *stream);
+
 /* Device plumbing for callers without a HIP runtime of their own (the library's runtime owns the device).
  * Copies are synchronous; kind 0 host to device, 1 device to host, 2 device to device. */
 int pharos_synth_device_count(void);

@@ -61,7 +61,26 @@ def lib():
         return _lib
     if not os.path.exists(LIB):
         build()
-    L = C.CDLL(LIB)
+    _lib = load(LIB)
+    return _lib
+
+
+def build_native(out_dir):
+    """Builds the oracle with -march=native into out_dir (the CPU baseline's build for the host it runs on) and
+    returns the library's path."""
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "native", f"OUT={out_dir}"], check=True)
+    return os.path.join(out_dir, "liboracle_native.so")
+
+
+def use(path):
+    """Makes the oracle library at `path` the one this module calls."""
+    global _lib
+    _lib = load(path)
+    return _lib
+
+
+def load(path):
+    L = C.CDLL(path)
     P, VP, U8P = C.POINTER, C.c_void_p, C.POINTER(C.c_uint8)
     sig = {
         "orc_init": (C.c_int, [C.c_char_p]),
@@ -105,7 +124,6 @@ def lib():
         f.argtypes = args
     if L.orc_init(DATA.encode()) != 0:
         raise RuntimeError(f"oracle could not load the Galileo code blobs from {DATA}")
-    _lib = L
     return L
 
 

@@ -48,6 +48,11 @@ class SynthTuning(C.Structure):
                 ("flags", C.c_uint32)]
 
 
+class PrecisionReport(C.Structure):
+    _fields_ = [("carrier_cycles", C.c_double), ("carrier_radians", C.c_double), ("magnitude", C.c_double),
+                ("code_entries", C.c_double), ("rows", C.c_uint64), ("reserved", C.c_uint64)]
+
+
 class SignalConstants(C.Structure):
     _fields_ = [("system", C.c_uint32), ("code_ms", C.c_uint32), ("entries_per_chip", C.c_uint32),
                 ("entries_per_ms", C.c_uint32), ("entries_per_period", C.c_uint32), ("value_count", C.c_uint32),
@@ -105,6 +110,7 @@ def lib():
         "pharos_synth_device_bytes": (U64, [VP]),
         "pharos_synth_window": (C.c_int, [VP, U64, U32, U32, VP, VP, U64, F32, VP, P(U64), VP]),
         "pharos_synth_window_static": (C.c_int, [VP, U64, U32, U32, VP, VP, U64, F32, VP, P(U64), VP]),
+        "pharos_synth_precision": (C.c_int, [VP, VP, U64, P(PrecisionReport), P(C.c_double), VP]),
         "pharos_synth_agc_account": (C.c_int, [P(SynthAgc), U64, U64, U64, P(SynthAdjustment), U32]),
         "pharos_synth_batch": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP]),
         "pharos_synth_batch_submit": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP, P(VP)]),

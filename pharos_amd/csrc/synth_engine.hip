@@ -240,19 +240,24 @@ __global__ __launch_bounds__(256) void row_prep_kernel(const pharos_synth_row *r
 // bounded: it gives up after wait_limit polls, when another tile has given up, or when the host sets the batch's
 // abort word; then it raises ERR_WAIT or ERR_ABORT and the tile goes on with the gain word as it stands (1.0 if
 // none was published), so every workgroup drains and the launch ends.
+// The checks sit off the common path: a wait that ends within 15 polls runs only the gain loads, the give-up word
+// is read every 16th poll, and the host's word (a read across the host link) every 256th once a wait has passed
+// 1024 polls.
 __device__ uint32_t wait_gain(const BatchState &S, uint32_t s) {
     unsigned long long w = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (uint32_t polls = 0; !(w & READY_BIT); polls++) {
-        uint32_t why = 0;
-        if (__hip_atomic_load(S.gave_up, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) why = ERR_WAIT;
-        else if (polls >= S.wait_limit) why = ERR_WAIT;
-        else if (S.abort_word && (polls & 63u) == 0 &&
-                 __hip_atomic_load(S.abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
-            why = ERR_ABORT;
-        if (why) {
-            atomicOr(S.err, why);
-            __hip_atomic_store(S.gave_up, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return __float_as_uint(1.0f);
+        if ((polls & 15u) == 15u) {
+            uint32_t why = 0;
+            if (__hip_atomic_load(S.gave_up, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || polls >= S.wait_limit)
+                why = ERR_WAIT;
+            else if (S.abort_word && polls >= 1024u && (polls & 255u) == 255u &&
+                     __hip_atomic_load(S.abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+                why = ERR_ABORT;
+            if (why) {
+                atomicOr(S.err, why);
+                __hip_atomic_store(S.gave_up, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return __float_as_uint(1.0f);
+            }
         }
         __builtin_amdgcn_s_sleep(8);
         w = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1479,6 +1484,125 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
     }
     if (status) return status;
     if (write_errno) { set_error("writing %s: %s", path, strerror(write_errno)); return ST_INTERNAL; }
+    return ST_OK;
+}
+
+// ------------------------------------------------------------------ precision diagnostic
+// The f32 rotor against FP64 over every sample of every row, and the Q32.32 code position against the exact
+// f64 position.  Each (block, lane) runs the synthesis kernel's rotor step for step: the f64 block seed
+// (sincos64, rotor.zig:37-38) times the f64 lane power, rounded to f32, then c <- c * w16 in f32 (rotor.zig:52-58).
+// Its reference is the exact phasor exp(2 pi i (phase0 + n step)) in f64, started from a library sincos at the
+// lane's first sample and advanced by the f64 16-step rotation.  Per sample, |Im(c conj(e))| is the sine of the
+// phase error (|c| is 1 within 1e-5), and |c|^2 - 1 twice the magnitude error.  The code position's error is
+// affine in the sample index, so its maximum over an epoch is at the first or last sample.
+constexpr uint32_t PREC_THREADS = 256;
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *p, double v) {
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));  // positive doubles order as their bits
+}
+
+__global__ __launch_bounds__(PREC_THREADS) void precision_kernel(const DevEngine *Ep, const pharos_synth_row *rows,
+                                                                 uint64_t nrows, unsigned long long *out) {
+    __shared__ RowPrep P;
+    __shared__ double red[2][PREC_THREADS];
+    const DevEngine E = load_engine(Ep);
+    const uint64_t r = blockIdx.x;
+    if (r >= nrows) return;
+    const pharos_synth_row row = rows[r];
+    if (row.table == PHAROS_SYNTH_INACTIVE || row.table >= E.ntables) return;
+    if (threadIdx.x == 0) row_prep(row, E.turn, P);
+    __syncthreads();
+    const double step = row.carrier_step;
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    double w16r, w16i;
+    sincospi(2.0 * ((16.0 * step) - floor(16.0 * step)), &w16i, &w16r);
+    double max_im = 0.0, max_mag = 0.0;
+    const uint32_t pairs = (E.spe + RESEED - 1) / RESEED * LANES16;
+    for (uint32_t q = threadIdx.x; q < pairs; q += PREC_THREADS) {
+        const uint32_t b = q / LANES16, l = q % LANES16, nb = b * RESEED;
+        double cc, cs;
+        sincos64(row.carrier_phase0 + (double)nb * step, E.turn, cc, cs);
+        float cre = (float)(cc * P.pw_c[l] - cs * P.pw_s[l]);
+        float cim = (float)(cc * P.pw_s[l] + cs * P.pw_c[l]);
+        const float wre = P.w_re, wim = P.w_im, nwim = -wim;
+        const double ph = row.carrier_phase0 + (double)(nb + l) * step;
+        double er, ei;
+        sincospi(2.0 * (ph - floor(ph)), &ei, &er);
+        for (uint32_t k = 0; k < STEPS && nb + l + LANES16 * k < E.spe; k++) {
+            const double xr = (double)cre, xi = (double)cim;
+            const double im = xi * er - xr * ei;
+            max_im = fmax(max_im, fabs(im));
+            max_mag = fmax(max_mag, fabs(xr * xr + xi * xi - 1.0));
+            const float t = fmaf(cre, wre, cim * nwim);
+            cim = fmaf(cre, wim, cim * wre);
+            cre = t;
+            const double nr = er * w16r - ei * w16i;
+            ei = er * w16i + ei * w16r;
+            er = nr;
+        }
+    }
+    red[0][threadIdx.x] = max_im;
+    red[1][threadIdx.x] = max_mag;
+    __syncthreads();
+    for (uint32_t h = PREC_THREADS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + h]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomic_max_pos(&out[0], asin(fmin(red[0][0], 1.0)) / two_pi);
+        atomic_max_pos(&out[1], 0.5 * red[1][0]);
+        // chip.Edge.of (chip.zig:17-24): first + phase_q / 2^32 + n eps_q / 2^32 against entry_phase0 + n eps.
+        const double fl = floor(row.entry_phase0);
+        const double phase_q = (double)frac_q(row.entry_phase0 - fl);
+        const double eps_q = round(row.entries_per_sample * 4294967296.0);
+        const double e0 = (fl - row.entry_phase0) + phase_q * 0x1p-32;
+        const double d = eps_q * 0x1p-32 - row.entries_per_sample;
+        const double last = e0 + (double)(E.spe - 1) * d;
+        atomic_max_pos(&out[2 + row.table], fmax(fabs(e0), fabs(last)));
+    }
+}
+
+extern "C" int pharos_synth_precision(pharos_synth_engine *e, const pharos_synth_row *rows, uint64_t nrows,
+                                      pharos_synth_precision_report *report, double *code_error_per_table,
+                                      void *stream_) {
+    g_error[0] = 0;
+    if (!e || !report || (nrows && !rows)) { set_error("null argument"); return ST_ARG; }
+    memset(report, 0, sizeof *report);
+    hipStream_t stream = (hipStream_t)stream_;
+    HIP_TRY(hipSetDevice(e->device), ST_INTERNAL);
+    const uint32_t nt = e->view.ntables;
+    const size_t words = 2 + (size_t)nt;
+    unsigned long long *d_out = nullptr;
+    HIP_TRY(hipMalloc((void **)&d_out, words * sizeof(unsigned long long)), ST_NOMEM);
+    std::vector<unsigned long long> h(words, 0);
+    int status = ST_OK;
+    if (hipMemcpyAsync(d_out, h.data(), words * sizeof(unsigned long long), hipMemcpyHostToDevice, stream) != hipSuccess)
+        status = ST_INTERNAL;
+    for (uint64_t done = 0; status == ST_OK && done < nrows;) {
+        const uint64_t n = std::min<uint64_t>(nrows - done, 1u << 30);
+        hipLaunchKernelGGL(precision_kernel, dim3((uint32_t)n), dim3(PREC_THREADS), 0, stream, e->d_engine, rows + done, n,
+                           d_out);
+        if (hipGetLastError() != hipSuccess) status = ST_INTERNAL;
+        done += n;
+    }
+    if (status == ST_OK &&
+        (hipMemcpyAsync(h.data(), d_out, words * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+         hipStreamSynchronize(stream) != hipSuccess))
+        status = ST_INTERNAL;
+    (void)hipFree(d_out);
+    if (status) { set_error("precision check failed on the device"); return status; }
+    auto as_double = [](unsigned long long b) { double v; memcpy(&v, &b, sizeof v); return v; };
+    report->carrier_cycles = as_double(h[0]);
+    report->carrier_radians = report->carrier_cycles * 2.0 * 3.14159265358979323846;
+    report->magnitude = as_double(h[1]);
+    report->rows = nrows;
+    for (uint32_t t = 0; t < nt; t++) {
+        const double v = as_double(h[2 + t]);
+        report->code_entries = std::max(report->code_entries, v);
+        if (code_error_per_table) code_error_per_table[t] = v;
+    }
     return ST_OK;
 }
 
