@@ -207,8 +207,8 @@ void pharos_synth_graph_destroy(pharos_synth_graph *graph);
  * device while earlier ones are copied to pinned host memory and written at their own file offsets by writer
  * threads.  The AGC runs between blocks as Run.pull does, and the job's agc, adjustments and clipped receive the
  * results as after pharos_synth_batch_wait.  job->out is not used (the sink holds four blocks of device and pinned
- * memory).  *bytes_written is the file's length when every write succeeded; a write error gives status 4, its
- * message and 0. */
+ * memory).  *bytes_written counts the bytes on file in order from its start (every block's, when all writes
+ * succeed); a write error gives status 4 and its message, with *bytes_written the part written before it. */
 int pharos_synth_to_file(const pharos_synth_job *job, const char *path, uint64_t *bytes_written);
 
 /* The .tag sidecar of a raw sample file (output/tag.zig:12-34, behind pharos_run_tag_text): seven lines naming the
@@ -233,6 +233,12 @@ typedef struct pharos_synth_precision_report {
 } pharos_synth_precision_report;
 int pharos_synth_precision(pharos_synth_engine *engine, const pharos_synth_row *rows, uint64_t nrows,
                            pharos_synth_precision_report *report, double *code_error_per_table, void *stream);
+
+/* A 64-bit checksum of `bytes` bytes of device memory (8-byte aligned), for comparing large outputs without copying
+ * them to the host; no counterpart in the reference.  With w_i the little-endian 64-bit words of the data (the last
+ * one zero-padded) and fmix MurmurHash3's 64-bit finaliser, it is fmix(S ^ bytes), S = sum over i of
+ * fmix(w_i ^ i * 0x9E3779B97F4A7C15) mod 2^64.  Not a cryptographic hash. */
+int pharos_synth_digest(const void *data, uint64_t bytes, uint64_t *digest, void *stream);
 
 /* Device plumbing for callers without a HIP runtime of their own (the library's runtime owns the device).
  * Copies are synchronous; kind 0 host to device, 1 device to host, 2 device to device. */

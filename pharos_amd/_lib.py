@@ -111,6 +111,7 @@ def lib():
         "pharos_synth_window": (C.c_int, [VP, U64, U32, U32, VP, VP, U64, F32, VP, P(U64), VP]),
         "pharos_synth_window_static": (C.c_int, [VP, U64, U32, U32, VP, VP, U64, F32, VP, P(U64), VP]),
         "pharos_synth_precision": (C.c_int, [VP, VP, U64, P(PrecisionReport), P(C.c_double), VP]),
+        "pharos_synth_digest": (C.c_int, [VP, U64, P(U64), VP]),
         "pharos_synth_agc_account": (C.c_int, [P(SynthAgc), U64, U64, U64, P(SynthAdjustment), U32]),
         "pharos_synth_batch": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP]),
         "pharos_synth_batch_submit": (C.c_int, [P(SynthJob), U32, P(SynthTuning), VP, P(VP)]),

@@ -931,6 +931,8 @@ struct SpareBlock { int device; void *p; size_t n; };
 static std::mutex g_spare_lock;
 static std::vector<SpareBlock> g_spare;
 constexpr size_t SPARE_MAX = 4, SPARE_MIN_BYTES = 1 << 20;
+constexpr size_t SPARE_MAX_TOTAL = (size_t)4 << 30;   // bytes the cache may hold in all
+constexpr size_t SPARE_POW2_LIMIT = (size_t)256 << 20; // above this, sizes round to multiples of it instead
 // Pinned host images, cached the same way (a pinned allocation costs far more than a small batch's copies).
 struct SparePinned { uint8_t *p; size_t n; };
 static std::vector<SparePinned> g_spare_pinned;
@@ -962,8 +964,12 @@ static void batch_free(pharos_synth_batch_handle *h) {
     abort_release(h->abort_slot);
     for (size_t i = 0; i < h->owned.size(); i++) {
         std::lock_guard<std::mutex> guard(g_spare_lock);
-        if (g_spare.size() < SPARE_MAX) g_spare.push_back({h->device, h->owned[i], h->owned_bytes[i]});
-        else (void)hipFree(h->owned[i]);
+        size_t held = 0;
+        for (const SpareBlock &b : g_spare) held += b.n;
+        if (g_spare.size() < SPARE_MAX && held + h->owned_bytes[i] <= SPARE_MAX_TOTAL)
+            g_spare.push_back({h->device, h->owned[i], h->owned_bytes[i]});
+        else
+            (void)hipFree(h->owned[i]);
     }
     if (h->pinned) {
         std::lock_guard<std::mutex> guard(g_spare_lock);
@@ -987,8 +993,10 @@ static int batch_alloc(pharos_synth_batch_handle *h, void **p, size_t n) {
         }
     }
     // Sizes round up to a power of two, so that a serving loop's batches of varying size reuse cached blocks.
+    // Large ones round to multiples of SPARE_POW2_LIMIT, so a big batch's pre-pass does not nearly double.
     size_t bytes = SPARE_MIN_BYTES;
-    while (bytes < n) bytes *= 2;
+    while (bytes < n && bytes < SPARE_POW2_LIMIT) bytes *= 2;
+    if (bytes < n) bytes = (n + SPARE_POW2_LIMIT - 1) / SPARE_POW2_LIMIT * SPARE_POW2_LIMIT;
     hipError_t st = hipMalloc(p, bytes);
     if (st != hipSuccess) { set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(st)); return ST_NOMEM; }
     h->owned.push_back(*p);
@@ -1360,20 +1368,25 @@ struct SinkResources {
     }
 };
 
-// Writes all of buf at offset; 0 or the errno of the failure.
-static int write_at(int fd, const uint8_t *buf, size_t n, uint64_t offset) {
-    while (n) {
-        const ssize_t w = pwrite(fd, buf, n, (off_t)offset);
+// The outcome of one block's write: 0 or the errno of the failure, and the bytes on file from the block's start.
+struct Written {
+    int err;
+    size_t bytes;
+};
+
+// Writes all of buf at offset.
+static Written write_at(int fd, const uint8_t *buf, size_t n, uint64_t offset) {
+    size_t done = 0;
+    while (done < n) {
+        const ssize_t w = pwrite(fd, buf + done, n - done, (off_t)(offset + done));
         if (w < 0) {
             if (errno == EINTR) continue;
-            return errno;
+            return {errno, done};
         }
-        if (w == 0) return EIO;
-        buf += w;
-        n -= (size_t)w;
-        offset += (uint64_t)w;
+        if (w == 0) return {EIO, done};
+        done += (size_t)w;
     }
-    return 0;
+    return {0, done};
 }
 
 extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path, uint64_t *bytes_written) {
@@ -1412,10 +1425,19 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
         G.components = J->agc->components;
     }
     std::vector<DevAdjust> log(std::max<uint32_t>(J->adjustments_cap, 1));
-    std::future<int> writing[SINK_SLOTS];  // per slot: errno of its block's write, 0 when written
+    std::future<Written> writing[SINK_SLOTS];  // per slot: the outcome of its block's write
+    size_t slot_block[SINK_SLOTS] = {};
+    // Per block, in file order: its length and the bytes of it on file (-1: not yet known).
+    std::vector<size_t> blk_len;
+    std::vector<int64_t> blk_on_file;
     uint64_t clip_total = 0, offset = 0;
     int status = ST_OK, write_errno = 0;
     uint64_t clip[100];
+    auto collect = [&](int k) {
+        const Written w = writing[k].get();
+        blk_on_file[slot_block[k]] = (int64_t)w.bytes;
+        if (w.err && !write_errno) write_errno = w.err;
+    };
     for (uint64_t done = 0, block = 0; done < J->epochs; block++) {
         const int k = (int)(block % SINK_SLOTS);
         const uint64_t abs_e = J->first_epoch + done;
@@ -1423,7 +1445,7 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
         const float gain = J->gain_mode == PHAROS_SYNTH_GAIN_CONSTANT ? J->constant_gain : (float)G.gain;
         uint8_t *d_slot = R.d_ring + (size_t)k * block_bytes, *h_slot = R.h_ring + (size_t)k * block_bytes;
         if (writing[k].valid()) {  // the slot's previous block: after its write, its buffers are free
-            write_errno = writing[k].get();
+            collect(k);
             if (write_errno) break;
         }
         status = pharos_synth_window(e, abs_e, n, J->channels, J->rows + done * J->channels, J->masks, J->mask_len,
@@ -1439,15 +1461,19 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
         const hipEvent_t ev = R.copied[k];
         const int fd = R.fd;
         const uint64_t at = offset;
-        auto write_block = [ev, fd, h_slot, nbytes, at]() -> int {
-            if (hipEventSynchronize(ev) != hipSuccess) return EIO;
+        auto write_block = [ev, fd, h_slot, nbytes, at]() -> Written {
+            if (hipEventSynchronize(ev) != hipSuccess) return {EIO, 0};
             return write_at(fd, h_slot, nbytes, at);
         };
+        slot_block[k] = blk_len.size();
+        blk_len.push_back(nbytes);
+        blk_on_file.push_back(-1);
         try {
             writing[k] = std::async(std::launch::async, write_block);
         } catch (...) {  // no thread to be had: write this block here
-            write_errno = write_block();
-            if (write_errno) break;
+            const Written w = write_block();
+            blk_on_file.back() = (int64_t)w.bytes;
+            if (w.err) { write_errno = w.err; break; }
         }
         offset += nbytes;
         uint64_t sum = 0;
@@ -1457,13 +1483,17 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
             agc_step(G, sum, (unsigned long long)n * e->view.spe * 2ull, abs_e + n, log.data(), (uint32_t)log.size());
         done += n;
     }
-    for (auto &w : writing)
-        if (w.valid()) {
-            const int r = w.get();
-            if (!write_errno) write_errno = r;
-        }
-    // The bytes on file: everything submitted, when every write succeeded (a failed write leaves a hole).
-    if (bytes_written) *bytes_written = write_errno ? 0 : offset;
+    for (int k = 0; k < SINK_SLOTS; k++)
+        if (writing[k].valid()) collect(k);
+    // The bytes on file in order from the start, as the reference's writer position counts them (output/sink.zig):
+    // whole blocks up to the first one not written in full, plus what of that one reached the file.
+    uint64_t prefix = 0;
+    for (size_t b = 0; b < blk_len.size(); b++) {
+        if (blk_on_file[b] < 0) break;
+        prefix += (uint64_t)blk_on_file[b];
+        if ((size_t)blk_on_file[b] < blk_len[b]) break;
+    }
+    if (bytes_written) *bytes_written = prefix;
     if (J->clipped) *J->clipped = clip_total;
     if (J->gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) {
         const uint32_t prior = J->agc->adjustments;
@@ -1484,6 +1514,61 @@ extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path,
     }
     if (status) return status;
     if (write_errno) { set_error("writing %s: %s", path, strerror(write_errno)); return ST_INTERNAL; }
+    return ST_OK;
+}
+
+// ------------------------------------------------------------------ digest
+// MurmurHash3's 64-bit finaliser.
+__host__ __device__ inline uint64_t fmix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    return x ^ (x >> 33);
+}
+
+constexpr uint32_t DIGEST_THREADS = 256;
+__global__ __launch_bounds__(DIGEST_THREADS) void digest_kernel(const uint8_t *data, uint64_t bytes,
+                                                                unsigned long long *sum) {
+    const uint64_t words = (bytes + 7) / 8, full = bytes / 8;
+    const gptr<uint64_t> w64 = gmem(reinterpret_cast<const uint64_t *>(data));
+    unsigned long long acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * DIGEST_THREADS + threadIdx.x; i < words;
+         i += (uint64_t)gridDim.x * DIGEST_THREADS) {
+        uint64_t w = 0;
+        if (i < full) {
+            w = w64[i];
+        } else {  // the last, partial word, zero-padded
+            for (uint64_t k = 8 * i; k < bytes; k++) w |= (uint64_t)gmem(data)[k] << (8 * (k - 8 * i));
+        }
+        acc += fmix64(w ^ (i * 0x9E3779B97F4A7C15ull));
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63u) == 0) atomicAdd(sum, acc);
+}
+
+extern "C" int pharos_synth_digest(const void *data, uint64_t bytes, uint64_t *digest, void *stream_) {
+    g_error[0] = 0;
+    if (!digest || (bytes && !data)) { set_error("null argument"); return ST_ARG; }
+    if ((uintptr_t)data % 8) { set_error("data must be 8-byte aligned"); return ST_ARG; }
+    hipStream_t stream = (hipStream_t)stream_;
+    unsigned long long *d_sum = nullptr, h_sum = 0;
+    HIP_TRY(hipMalloc((void **)&d_sum, sizeof *d_sum), ST_NOMEM);
+    int status = ST_OK;
+    if (hipMemcpyAsync(d_sum, &h_sum, sizeof h_sum, hipMemcpyHostToDevice, stream) != hipSuccess) status = ST_INTERNAL;
+    if (status == ST_OK && bytes) {
+        const uint64_t words = (bytes + 7) / 8;
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((words + DIGEST_THREADS - 1) / DIGEST_THREADS, 8192);
+        hipLaunchKernelGGL(digest_kernel, dim3(blocks), dim3(DIGEST_THREADS), 0, stream, (const uint8_t *)data, bytes,
+                           d_sum);
+        if (hipGetLastError() != hipSuccess) status = ST_INTERNAL;
+    }
+    if (status == ST_OK && (hipMemcpyAsync(&h_sum, d_sum, sizeof h_sum, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+                            hipStreamSynchronize(stream) != hipSuccess))
+        status = ST_INTERNAL;
+    (void)hipFree(d_sum);
+    if (status) { set_error("digest failed on the device"); return status; }
+    *digest = fmix64(h_sum ^ bytes);
     return ST_OK;
 }
 

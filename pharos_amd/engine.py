@@ -62,6 +62,35 @@ class DeviceBuffer:
             pass
 
 
+def digest(ptr, nbytes, stream=None):
+    """pharos_synth_digest of nbytes of device memory at ptr."""
+    d = C.c_uint64()
+    check(_lib.lib().pharos_synth_digest(ptr, int(nbytes), C.byref(d), stream.handle if stream else None))
+    return int(d.value)
+
+
+def host_digest(data):
+    """The same checksum of host bytes (numpy), for comparing device outputs with host ones."""
+    b = np.frombuffer(bytes(data) if not isinstance(data, np.ndarray) else data.tobytes(), dtype=np.uint8)
+    n = len(b)
+    pad = np.zeros((n + 7) // 8 * 8, dtype=np.uint8)
+    pad[:n] = b
+    w = pad.view("<u8")
+    m = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+    def fmix(x):
+        x = x ^ (x >> np.uint64(33))
+        x = x * np.uint64(0xff51afd7ed558ccd)
+        x = x ^ (x >> np.uint64(33))
+        x = x * np.uint64(0xc4ceb9fe1a85ec53)
+        return x ^ (x >> np.uint64(33))
+
+    with np.errstate(over="ignore"):
+        i = np.arange(len(w), dtype=np.uint64)
+        s = int(fmix(w ^ (i * np.uint64(0x9E3779B97F4A7C15))).sum(dtype=np.uint64)) & int(m)
+        return int(fmix(np.array([s ^ n], dtype=np.uint64))[0])
+
+
 class Stream:
     def __init__(self, device=0):
         s = C.c_void_p()

@@ -29,8 +29,11 @@ from .engine import Engine, DeviceBuffer, Stream, require_gpu
 
 SLICE_MIN = 1000        # epochs of a request per iteration, at least (ten AGC windows) ...
 SLICE_WORK = 8e9        # ... or about this many channel-samples (a few ms of the GPU), in whole AGC windows
-RESERVE = 24 << 30      # HBM left free for the library's own allocations: engines, and the batch state it caches
-                        # (up to four blocks, each the power of two above a batch's tables and row pre-pass)
+RESERVE = 8 << 30       # HBM left free for the library's own allocations: engines and the batch-state blocks it
+                        # caches between batches (at most 4 GiB of them); each request's share of a batch's state
+                        # is held against the arena while it is admitted (Request.state_bytes)
+PREP_BYTES = 272        # the batched form's row pre-pass per row (RowPrep) of a job with several tiles per epoch
+STATE_SLACK = 2         # a batch's state block rounds up to a power of two below 256 MiB
 
 
 class Request:
@@ -70,6 +73,14 @@ class Request:
         s, chans = self.scenario.spec()
         masks = int(L.lib().pharos_scenario_mask_bound(C_.byref(s), chans, self.slice_epochs)) + 16
         return self.slice_epochs * max(1, len(self.scenario.channels)) * 64, masks
+
+    def state_bytes(self):
+        """Device bytes of this request's share of a batch's state: the row pre-pass of one slice (when an epoch
+        spans several tiles) and its job and segment records, with the rounding of the block they live in."""
+        rows = self.slice_epochs * max(1, len(self.scenario.channels))
+        prep = rows * PREP_BYTES if self.scenario.samples_per_epoch > 4096 else 0
+        segments = self.slice_epochs // 100 + 2
+        return STATE_SLACK * (prep + 256 + segments * 64)
 
     def buffer_sizes(self):
         """Bytes of the request's device buffers: its output, then two (rows, masks) pairs."""
@@ -159,6 +170,17 @@ class DeviceArena:
         self.blocks = []        # DeviceBuffer per block
         self.ranges = []        # per block: sorted [offset, size] free ranges
         self.used = 0
+        self.held = 0           # bytes held against the limit for memory the library allocates outside the arena
+
+    def hold(self, nbytes):
+        """Holds nbytes of the limit for an allocation outside the arena; False if the limit has no room."""
+        if self.used + self.held + int(nbytes) > self.limit:
+            return False
+        self.held += int(nbytes)
+        return True
+
+    def unhold(self, nbytes):
+        self.held -= int(nbytes)
 
     def _round(self, n):
         return (max(int(n), 1) + self.ALIGN - 1) // self.ALIGN * self.ALIGN
@@ -169,6 +191,8 @@ class DeviceArena:
     def alloc(self, nbytes):
         """An ArenaBuffer of at least nbytes, or None if it does not fit within the limit."""
         n = self._round(nbytes)
+        if self.used + self.held + n > self.limit:
+            return None
         for bi, free in enumerate(self.ranges):
             for k, (off, size) in enumerate(free):
                 if size >= n:
@@ -179,9 +203,9 @@ class DeviceArena:
                     self.used += n
                     return ArenaBuffer(self, bi, off, n)
         size = max(self.block_bytes, n)
-        if self.reserved() + size > self.limit:
+        if self.reserved() + self.held + size > self.limit:
             size = n
-            if self.reserved() + size > self.limit:
+            if self.reserved() + self.held + size > self.limit:
                 return None
         self.blocks.append(DeviceBuffer(size, self.device))
         self.ranges.append([[n, size - n]] if size > n else [])
@@ -239,9 +263,13 @@ class Server:
         self.phase_s = collections.Counter()  # host seconds per loop phase
 
     def _admit(self, req):
-        """Places the request's buffers in the arena; False (and nothing held) if they do not fit now."""
+        """Places the request's buffers in the arena and holds its share of the batch state; False (and nothing
+        held) if they do not fit now."""
+        if not self.arena.hold(req.state_bytes()):
+            return False
         got = self.arena.alloc_all(req.buffer_sizes())
         if got is None:
+            self.arena.unhold(req.state_bytes())
             return False
         req.out = got[0]
         # Two row/mask buffers per request: the next slice is written into one while the batch reads the other.
@@ -257,6 +285,7 @@ class Server:
         return b
 
     def _retire(self, req, keep_output):
+        self.arena.unhold(req.state_bytes())
         for pair in req.buffers:
             for b in pair:
                 b.free()

@@ -63,3 +63,25 @@ def test_close_returns_blocks(arena):
     blocks = list(arena.blocks)
     arena.close()
     assert all(b.freed for b in blocks) and arena.used == 0 and not arena.blocks
+
+
+def test_held_state_counts_against_the_limit(arena):
+    # 1 MiB limit: holding 900 KiB for a batch's state leaves room for 124 KiB of buffers, no more.
+    assert arena.hold(900 << 10)
+    assert arena.alloc(200 << 10) is None
+    b = arena.alloc(100 << 10)
+    assert b is not None
+    assert not arena.hold(100 << 10)
+    arena.unhold(900 << 10)
+    assert arena.hold(500 << 10) and arena.alloc(300 << 10) is not None
+
+
+def test_request_state_share_covers_the_row_prepass():
+    from pharos_amd.scenario import c3
+    for sc in c3(seed=4, count=40):
+        r = serve.Request(sc)
+        rows = r.slice_epochs * len(sc.channels)
+        prep = rows * serve.PREP_BYTES if sc.samples_per_epoch > 4096 else 0
+        # the share covers the pre-pass with the block's rounding (below 256 MiB a power of two: at most 2x)
+        assert r.state_bytes() >= 2 * prep
+        assert r.state_bytes() <= 2 * prep + 64 * (r.slice_epochs // 100 + 2) * 2 + 1024
