@@ -120,6 +120,36 @@ uint64_t pharos_synth_bytes_per_epoch(const pharos_synth_engine *engine);
 /* Device bytes the engine holds (tables), for admission by free memory. */
 uint64_t pharos_synth_device_bytes(const pharos_synth_engine *engine);
 
+/* The host's epoch-rate hand-off: per channel and block of epochs, the carrier phase and the signal delay as cubic
+ * polynomials of the receive time, and the amplitude as a linear one, instead of one row per channel and epoch.  The
+ * device evaluates them in f64 at every epoch edge and derives the rows (parameters.derive, parameters.zig:51-81),
+ * so the host's per-epoch work moves to the GPU.  tau is the receive time in ms from the block's first epoch edge. */
+typedef struct pharos_synth_segment {
+    double phase[4];             /* carrier phase, cycles: ((phase[3] tau + phase[2]) tau + phase[1]) tau + phase[0] */
+    double delay[4];             /* signal delay, ms (transmit time = receive time - delay), the same form */
+    double amp[2];               /* amplitude amp[1] tau + amp[0], rounded to f32 per epoch (parameters.zig:78-81) */
+    double lead_entries;         /* entries the leading component is ahead; 0 without a lead */
+    int64_t if_hz;               /* carrier offset from the band centre */
+    int64_t base_offset_ms;      /* GpsTime.baseMs (domain/time/gps.zig:109-115): the channel's time scale minus GPS ms */
+    uint64_t mask_first;         /* masks[mask_offset + k - mask_first] is the mask byte of code period k ... */
+    uint32_t mask_offset;
+    uint32_t mask_count;         /* ... for mask_first <= k < mask_first + mask_count */
+    uint32_t code_ms;            /* code period, ms */
+    uint32_t entries_per_ms;
+    uint32_t table;              /* PHAROS_SYNTH_INACTIVE: no rows for this channel and block */
+    uint8_t lead_bits;
+    uint8_t reserved[3];
+} pharos_synth_segment;          /* 136 bytes */
+
+/* Rows [epochs][channels] (device) of epochs first_epoch .. first_epoch + epochs - 1 from segments
+ * [blocks][channels] (device), block b covering epochs first_epoch + b * block_epochs onwards.  Epoch 0 of the run
+ * starts at GPS time run_start_ms + run_start_sub_ms (sub in [0, 1)).  A row is inactive where derive rejects the
+ * epoch (rate outside +-3 %, negative time) or its segment is; an active row's mask_offset and mask_count index the
+ * segment's mask range.  Stream-ordered: the rows are ready for any launch queued after it on `stream`. */
+int pharos_synth_derive(const pharos_synth_segment *segments, uint32_t block_epochs, uint64_t first_epoch,
+                        uint32_t epochs, uint32_t channels, int64_t run_start_ms, double run_start_sub_ms,
+                        uint32_t samples_per_epoch, pharos_synth_row *rows, void *stream);
+
 /* Plain form: one launch synthesises `epochs` epochs from first_epoch under one gain, like one window of
  * Run.runJob(.produce) (run.zig:367-380).  rows (device) is [epochs][channels]; masks (device) holds mask_len
  * bytes; out (device) receives epochs * bytes_per_epoch bytes.  clipped (host, optional) receives one count per

@@ -114,6 +114,7 @@ def load(path):
         "orc_rotor_values": (None, [C.c_double, C.c_double, C.c_uint32, C.c_uint32, VP, VP]),
         "orc_agc_account": (C.c_int, [P(Agc), C.c_uint64, C.c_uint64, C.c_uint64, P(C.c_double)]),
         "orc_db_linear": (C.c_float, [C.c_double]),
+        "orc_derive_segment": (C.c_int, [VP, C.c_int64, C.c_double, C.c_uint32, C.c_uint32, VP]),
         "orc_symbol_locate": (C.c_int, [C.c_int, C.c_int, C.c_uint64, P(C.c_int64), P(C.c_uint32), P(C.c_uint32)]),
         "orc_symbol_masks": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_uint32, VP, C.c_int64, C.c_uint32,
                                        C.c_uint32, VP]),
@@ -227,3 +228,16 @@ def symbol_masks(signal, prn, k0, n, frame_bits=None, frame0=0):
     if lib().orc_symbol_masks(SIGNAL[signal], prn, k0, n, ptr, frame0, frames, bpf, out.ctypes.data):
         raise ValueError(signal)
     return out[:n]
+
+
+def derive_segments(segments, run_start_ms, sub_ms, first_epoch, epochs, block_epochs, spe):
+    """Rows [epochs, channels] from segments [blocks, channels] (a numpy array of the segment record)."""
+    L = lib()
+    blocks, nch = segments.shape
+    rows = np.zeros((epochs, nch), dtype=ROW_DTYPE)
+    seg = np.ascontiguousarray(segments)
+    for e in range(epochs):
+        for c in range(nch):
+            L.orc_derive_segment(seg[e // block_epochs, c:c + 1].ctypes.data, run_start_ms + first_epoch + e, sub_ms,
+                                 e % block_epochs, spe, rows[e, c:c + 1].ctypes.data)
+    return rows

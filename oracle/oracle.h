@@ -104,6 +104,16 @@ typedef struct orc_endpoint { double phase_cycles; int64_t tx_ms; double tx_sub_
  * endpoint a on the channel's time scale (GpsTime.baseMs), supplied by the caller. */
 int orc_derive(const orc_channel_constants *ch, int64_t base_ms, const orc_endpoint *a, const orc_endpoint *b,
                int64_t t_rx_ms, double t_rx_sub_ms, uint32_t samples_per_epoch, double cn0_dbhz, orc_row *out);
+/* Per-block polynomial hand-off (same layout as pharos_synth_segment in include/pharos_synth.h, 136 bytes): one
+ * row from a segment at tau_ms epochs into its block, the epoch's receive time being t_rx_ms + sub. */
+typedef struct orc_segment {
+    double phase[4], delay[4], amp[2], lead_entries;
+    int64_t if_hz, base_offset_ms;
+    uint64_t mask_first;
+    uint32_t mask_offset, mask_count, code_ms, entries_per_ms, table;
+    uint8_t lead_bits, reserved[3];
+} orc_segment;
+int orc_derive_segment(const orc_segment *g, int64_t t_rx_ms, double sub, uint32_t tau_ms, uint32_t spe, orc_row *out);
 size_t orc_masks_needed(const orc_row *p, uint32_t samples_per_epoch, uint32_t entries_per_period);
 float orc_amplitude(double cn0_dbhz, uint32_t samples_per_epoch);
 

@@ -998,6 +998,48 @@ int orc_derive(const orc_channel_constants *ch, int64_t base_ms, const orc_endpo
     return 1;
 }
 
+/* A segment's cubic in Horner form, one rounding per operation. */
+static double seg_cubic(const double *c, double tau) {
+    double v = c[3];
+    v = v * tau + c[2];
+    v = v * tau + c[1];
+    return v * tau + c[0];
+}
+
+/* One epoch's row from a segment: the endpoints at receive times (t_rx_ms + sub) and one ms later, from the
+ * segment's cubics (transmit time = receive time - delay, split into whole ms and a fraction in [0, 1)), then
+ * parameters.derive (parameters.zig:51-70) as orc_derive restates it; the amplitude is f32 of the segment's linear
+ * amplitude (the reference's amplitude() rounds its f64 value to f32 the same way, parameters.zig:78-81). */
+int orc_derive_segment(const orc_segment *g, int64_t t_rx_ms, double sub, uint32_t tau_ms, uint32_t spe, orc_row *out) {
+    memset(out, 0, sizeof *out);
+    out->table = 0xFFFFFFFFu;
+    if (g->table == 0xFFFFFFFFu) return 0;
+    const double tau = (double)tau_ms;
+    orc_endpoint a, b;
+    double xa = seg_cubic(g->delay, tau) - sub, xb = seg_cubic(g->delay, tau + 1.0) - sub;
+    double ua = ceil(xa), ub = ceil(xb);
+    a.phase_cycles = seg_cubic(g->phase, tau);
+    a.tx_ms = t_rx_ms - (int64_t)ua;
+    a.tx_sub_ms = ua - xa;
+    b.phase_cycles = seg_cubic(g->phase, tau + 1.0);
+    b.tx_ms = t_rx_ms + 1 - (int64_t)ub;
+    b.tx_sub_ms = ub - xb;
+    orc_channel_constants ch = {g->if_hz, g->code_ms, g->entries_per_ms, 0, g->table};
+    if (!orc_derive(&ch, a.tx_ms + g->base_offset_ms, &a, &b, t_rx_ms, sub, spe, 0.0, out)) return 0;
+    out->amp = (float)(g->amp[1] * tau + g->amp[0]);
+    out->lead_bits = g->lead_bits;
+    out->lead_entries = g->lead_bits ? g->lead_entries : 0.0;
+    if (out->period0 >= g->mask_first && out->period0 < g->mask_first + g->mask_count) {
+        out->mask_offset = g->mask_offset + (uint32_t)(out->period0 - g->mask_first);
+        uint64_t left = g->mask_first + g->mask_count - out->period0;
+        out->mask_count = (uint16_t)(left < 65535 ? left : 65535);
+    } else {
+        out->mask_offset = g->mask_offset;
+        out->mask_count = 0;
+    }
+    return 1;
+}
+
 /* ================================================================ gain.zig */
 int orc_agc_account(orc_agc *agc, uint64_t epoch_after, uint64_t clipped, uint64_t components, double *rate_out) {
     agc->clipped += clipped; /* gain.zig:19-35 */

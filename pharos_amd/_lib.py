@@ -81,6 +81,12 @@ ROW_DTYPE = np.dtype([("carrier_phase0", "<f8"), ("carrier_step", "<f8"), ("entr
                       ("amp", "<f4"), ("mask_offset", "<u4"), ("mask_count", "<u2"), ("lead_bits", "u1"),
                       ("flags", "u1")])
 assert ROW_DTYPE.itemsize == 64
+# pharos_synth_segment (136 bytes, include/pharos_synth.h)
+SEGMENT_DTYPE = np.dtype([("phase", "<f8", 4), ("delay", "<f8", 4), ("amp", "<f8", 2), ("lead_entries", "<f8"),
+                          ("if_hz", "<i8"), ("base_offset_ms", "<i8"), ("mask_first", "<u8"), ("mask_offset", "<u4"),
+                          ("mask_count", "<u4"), ("code_ms", "<u4"), ("entries_per_ms", "<u4"), ("table", "<u4"),
+                          ("lead_bits", "u1"), ("reserved", "u1", 3)])
+assert SEGMENT_DTYPE.itemsize == 136
 INACTIVE = 0xFFFFFFFF
 
 SIGNALS = ["L1CA", "L1C", "L2C", "L5", "B1C", "B1I", "B2I", "B3I", "B2a", "B2b", "E1", "E5a", "E5b", "E6", "G1", "G2"]
@@ -152,6 +158,8 @@ def lib():
         "pharos_scenario_endpoint": (None, [P(ScenarioSpec), P(ChannelSpec), I64, P(C.c_double), P(I64), P(C.c_double)]),
         "pharos_derive": (C.c_int, [P(ScenarioSpec), P(ChannelSpec), U32, U64, VP]),
         "pharos_scenario_nav_bit": (C.c_int, [P(ChannelSpec), I64, U32]),
+        "pharos_scenario_segments": (I64, [P(ScenarioSpec), P(ChannelSpec), U64, U32, U32, VP, VP, U64]),
+        "pharos_synth_derive": (C.c_int, [VP, U32, U64, U32, U32, I64, C.c_double, U32, VP, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

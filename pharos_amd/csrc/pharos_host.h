@@ -78,6 +78,13 @@ uint64_t pharos_scenario_mask_bound(const pharos_scenario_spec *scenario, const 
 // One endpoint of a channel at receiver time t_rx_ms (exact ms): the carrier phase in cycles and the transmit time.
 void pharos_scenario_endpoint(const pharos_scenario_spec *scenario, const pharos_channel_spec *ch, int64_t t_rx_ms,
                               double *phase_cycles, int64_t *tx_ms, double *tx_sub_ms);
+// The same channels as segments (include/pharos_synth.h) for the device derive: per channel and block of block_epochs
+// epochs, cubic interpolants of the channel model's carrier phase and delay, the amplitude, and the block's symbol
+// masks (written to masks, at most mask_cap bytes).  segs is [blocks][channel_count]; channel c uses table c.
+// Returns the mask bytes used or a negative status.
+int64_t pharos_scenario_segments(const pharos_scenario_spec *scenario, const pharos_channel_spec *channels,
+                                 uint64_t first_epoch, uint32_t epochs, uint32_t block_epochs,
+                                 pharos_synth_segment *segs, uint8_t *masks, uint64_t mask_cap);
 // The stand-in's nav bit `bit` of frame `frame` of a channel (the message content SymbolSource reads; 0 or 1).
 int pharos_scenario_nav_bit(const pharos_channel_spec *ch, int64_t frame, uint32_t bit);
 // parameters.derive with the channel's constants; returns 1 for an active row.

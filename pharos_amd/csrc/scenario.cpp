@@ -69,22 +69,52 @@ uint8_t nav_bit(const pharos_channel_spec &c, uint64_t frame, uint64_t bit) {
 }
 
 // SymbolSource.fill (src/products/run/symbols/source.zig:53-89), with a random nav bit per (frame, bit).
+// The position (frame, bit, NH chip) of period k0 is found by division once; later periods step it by one code period
+// each, and a bit's nav value is drawn once per bit, not per period.
 void fill_masks(const pharos_signal_constants &k, const pharos_channel_spec &c, uint64_t k0, uint64_t n, const uint8_t *secondary,
                 uint32_t sec_len, uint8_t *out) {
     const Timing t = timing_of(k, c);
     const uint64_t bit_ms = (uint64_t)t.code_ms * t.nh_len;
+    const uint64_t ms0 = k0 * t.code_ms + t.offset_ms;
+    uint64_t frame = ms0 / t.frame_ms, in_frame = ms0 % t.frame_ms;
+    uint64_t bit = in_frame / bit_ms, in_bit = in_frame % bit_ms;
+    uint32_t sec_at = sec_len ? (uint32_t)(k0 % sec_len) : 0;
+    uint8_t nav = nav_bit(c, frame, bit);
+    uint32_t nh_pos = (uint32_t)(in_bit / t.code_ms);
     for (uint64_t i = 0; i < n; i++) {
-        const uint64_t kk = k0 + i;
-        const uint64_t ms = kk * t.code_ms + t.offset_ms;
-        const uint64_t in_frame = ms % t.frame_ms;
-        const uint32_t nh_pos = (uint32_t)((in_frame % bit_ms) / t.code_ms);
-        const uint64_t frame = ms / t.frame_ms, bit = in_frame / bit_ms;
-        const uint8_t nav = nav_bit(c, frame, bit);
         const uint8_t nh = (uint8_t)((t.nh_code >> nh_pos) & 1u);
         uint8_t m = (uint8_t)(nav ^ nh);
-        if (sec_len) m |= (uint8_t)(secondary[kk % sec_len] << 1);
+        if (sec_len) {
+            m |= (uint8_t)(secondary[sec_at] << 1);
+            if (++sec_at == sec_len) sec_at = 0;
+        }
         out[i] = m;
+        // The next period: in_frame moves on by one code period, wrapping into the next frame.
+        in_frame += t.code_ms;
+        in_bit += t.code_ms;
+        nh_pos++;
+        if (in_frame >= t.frame_ms) {
+            in_frame -= t.frame_ms;
+            frame++;
+            bit = in_frame / bit_ms;
+            in_bit = in_frame % bit_ms;
+            nh_pos = (uint32_t)(in_bit / t.code_ms);
+            nav = nav_bit(c, frame, bit);
+        } else if (in_bit >= bit_ms) {
+            in_bit -= bit_ms;
+            nh_pos = (uint32_t)(in_bit / t.code_ms);
+            bit++;
+            nav = nav_bit(c, frame, bit);
+        }
     }
+}
+
+// A segment's cubic in Horner form, one rounding per operation, as the device derive evaluates it.
+double cubic(const double *c, double tau) {
+    double v = c[3];
+    v = v * tau + c[2];
+    v = v * tau + c[1];
+    return v * tau + c[0];
 }
 
 // parameters.masksNeeded (parameters.zig:25-29).
@@ -115,6 +145,41 @@ Endpoint endpoint_at(const pharos_scenario_spec *s, const pharos_channel_spec *c
     const double d = range / LIGHT * 1000.0 - s->start_gps_sub_ms;  // travel time in ms, less the sub-ms part
     const double up = ceil(d);
     return {cycles, t_rx_ms - (int64_t)up, up - d};
+}
+
+// The channel model at receive time t0_ms + tau_ms (tau any real value): carrier phase in cycles and signal delay in
+// ms.  The time from the start is formed from the integer ms first, so tau keeps its precision.
+void model_at(const pharos_scenario_spec *s, const pharos_channel_spec *c, double lambda, int64_t t0_ms, double tau_ms,
+              double *phase, double *delay_ms) {
+    const double t = ((double)(t0_ms - s->start_gps_ms) + tau_ms) / 1000.0;
+    double wobble = 0.0;
+    if (c->wobble_accel_mps2 != 0.0 && c->wobble_period_s > 0.0) {
+        const double w = 2.0 * M_PI / c->wobble_period_s;
+        wobble = c->wobble_accel_mps2 / (w * w) * sin(w * t);
+    }
+    const double cycles = -(c->doppler_hz * t + 0.5 * c->doppler_rate_hz_s * t * t) + wobble / lambda;
+    *phase = cycles;
+    *delay_ms = (c->range0_m + cycles * lambda) / LIGHT * 1000.0;
+}
+
+// The cubic through (tau_j, y_j), j = 0..3, as monomial coefficients in tau (Newton's divided differences).
+void cubic_through(const double *tau, const double *y, double *coef) {
+    double d[4] = {y[0], y[1], y[2], y[3]};
+    for (int k = 1; k < 4; k++)
+        for (int j = 3; j >= k; j--) d[j] = (d[j] - d[j - 1]) / (tau[j] - tau[j - k]);
+    // p(x) = d0 + d1 (x - t0) + d2 (x - t0)(x - t1) + d3 (x - t0)(x - t1)(x - t2), expanded.
+    double c[4] = {d[3], 0.0, 0.0, 0.0};  // Horner over the nodes t2, t1, t0
+    for (int j = 2; j >= 0; j--) {
+        // c <- c * (x - tau_j) + d_j
+        double nc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < 3; k++) {
+            nc[k + 1] += c[k];
+            nc[k] -= c[k] * tau[j];
+        }
+        nc[0] += d[j];
+        for (int k = 0; k < 4; k++) c[k] = nc[k];
+    }
+    for (int k = 0; k < 4; k++) coef[k] = c[k];
 }
 
 double amplitude(const pharos_channel_spec *c, uint32_t spe) {
@@ -172,6 +237,73 @@ int pharos_derive(const pharos_scenario_spec *s, const pharos_channel_spec *c, u
     const int64_t ta = s->start_gps_ms + (int64_t)epoch;
     return derive_from(s, c, k, table, epoch, endpoint_at(s, c, lambda, ta), endpoint_at(s, c, lambda, ta + 1),
                        (float)amplitude(c, s->samples_per_epoch), out);
+}
+
+int64_t pharos_scenario_segments(const pharos_scenario_spec *s, const pharos_channel_spec *channels, uint64_t first_epoch,
+                                 uint32_t epochs, uint32_t block_epochs, pharos_synth_segment *segs, uint8_t *masks,
+                                 uint64_t mask_cap) {
+    if (!s || (s->channel_count && !channels) || !segs || epochs == 0 || block_epochs == 0) return -2;
+    const uint32_t nch = s->channel_count;
+    const uint32_t blocks = (epochs + block_epochs - 1) / block_epochs;
+    uint64_t used = 0;
+    std::vector<uint8_t> sec(4096);
+    for (uint32_t c = 0; c < nch; c++) {
+        const pharos_channel_spec &ch = channels[c];
+        pharos_signal_constants k;
+        if (pharos_signal_info(ch.signal, &k)) return -2;
+        const int sl = pharos_secondary_code(ch.signal, ch.prn, sec.data(), (uint32_t)sec.size());
+        if (sl < 0) return -2;
+        const double lambda = LIGHT / carrier_hz(k, ch);
+        const double amp = amplitude(&ch, s->samples_per_epoch);
+        const int64_t base_offset = k.system == 1 ? -BDS_OFFSET_MS
+                                  : (k.system == 3 ? -(int64_t)s->leap_seconds * 1000 + MOSCOW_OFFSET_MS : 0);
+        for (uint32_t b = 0; b < blocks; b++) {
+            pharos_synth_segment &g = segs[(size_t)b * nch + c];
+            memset(&g, 0, sizeof g);
+            const uint32_t e0 = b * block_epochs;
+            const uint32_t len = std::min<uint32_t>(block_epochs, epochs - e0);
+            const int64_t t0 = s->start_gps_ms + (int64_t)(first_epoch + e0);
+            // Cubic interpolants of the model through four receive times spread over the block's span.
+            double tau[4], yp[4], yd[4];
+            for (int j = 0; j < 4; j++) {
+                tau[j] = (double)len * j / 3.0;
+                model_at(s, &ch, lambda, t0, tau[j], &yp[j], &yd[j]);
+            }
+            cubic_through(tau, yp, g.phase);
+            cubic_through(tau, yd, g.delay);
+            g.amp[0] = amp;
+            g.amp[1] = 0.0;
+            g.if_hz = ch.if_hz;
+            g.base_offset_ms = base_offset;
+            g.code_ms = k.code_ms;
+            g.entries_per_ms = k.entries_per_ms;
+            g.table = c;
+            if (ch.lead_component && ch.lead_entries > 0.0) {  // channels.zig:353-356
+                g.lead_bits = (uint8_t)(ch.lead_component == 1 ? k.state_bits_pilot : k.state_bits_data);
+                g.lead_entries = g.lead_bits ? ch.lead_entries : 0.0;
+            }
+            // The code periods the block's rows can touch: from the transmit time at its first edge to the one at
+            // its last, one period of slack before, and after it masksNeeded's reach (a tile's run-past and the
+            // lead, parameters.zig:25-29) plus slack.
+            auto period_at = [&](uint32_t tau_ms) {
+                const double x = cubic(g.delay, (double)tau_ms) - s->start_gps_sub_ms;
+                const int64_t tx = t0 + (int64_t)tau_ms - (int64_t)ceil(x);
+                return div_floor(tx + base_offset, (int64_t)k.code_ms);
+            };
+            const int64_t p_lo = period_at(0) - 1;
+            const int64_t lead_periods = (int64_t)ceil(g.lead_entries / (double)k.entries_per_period);
+            const int64_t p_hi = period_at(len) + 3 + lead_periods;
+            if (p_lo < 0) { g.table = PHAROS_SYNTH_INACTIVE; continue; }  // before the time scale's origin
+            const uint64_t count = (uint64_t)(p_hi - p_lo);
+            if (used + count > mask_cap || count > 65535) return -3;
+            if (masks) fill_masks(k, ch, (uint64_t)p_lo, count, sec.data(), (uint32_t)sl, masks + used);
+            g.mask_first = (uint64_t)p_lo;
+            g.mask_offset = (uint32_t)used;
+            g.mask_count = (uint32_t)count;
+            used += count;
+        }
+    }
+    return (int64_t)used;
 }
 
 int pharos_scenario_nav_bit(const pharos_channel_spec *c, int64_t frame, uint32_t bit) {

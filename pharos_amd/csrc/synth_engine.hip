@@ -224,6 +224,98 @@ __global__ __launch_bounds__(AGC_THREADS) void agc_window_kernel(DevAgc *G, cons
     }
 }
 
+// ------------------------------------------------------------------ device derive
+// The segment's cubic at tau in Horner form, one rounding per operation (the host evaluates it the same way).
+__host__ __device__ inline double cubic(const double *c, double tau) {
+    double v = c[3];
+    v = v * tau + c[2];
+    v = v * tau + c[1];
+    return v * tau + c[0];
+}
+
+__host__ __device__ inline int64_t div_floor64(int64_t a, int64_t b) {
+    const int64_t q = a / b;
+    return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+__host__ __device__ inline int64_t mod_floor64(int64_t a, int64_t b) {
+    const int64_t r = a % b;
+    return r < 0 ? r + b : r;
+}
+
+// One row (epoch e, channel c): the endpoints at the epoch's two edges from the segment's polynomials, then
+// parameters.derive (parameters.zig:51-70) with ifPhase (:72-76) in the same operations and order.
+__global__ __launch_bounds__(256) void derive_kernel(const pharos_synth_segment *segs, uint32_t block_epochs,
+                                                     uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+                                                     int64_t start_ms, double sub, uint32_t spe, pharos_synth_row *rows) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)epochs * channels) return;
+    const uint32_t e = (uint32_t)(i / channels), c = (uint32_t)(i % channels);
+    const pharos_synth_segment &S = segs[(size_t)(e / block_epochs) * channels + c];
+    pharos_synth_row r;
+    memset(&r, 0, sizeof r);
+    r.table = PHAROS_SYNTH_INACTIVE;
+    const double tau = (double)(e % block_epochs);
+    const int64_t ta = start_ms + (int64_t)(first_epoch + e);
+    if (S.table != PHAROS_SYNTH_INACTIVE) {
+        // Endpoints: transmit time = (ms + sub) - delay, split into whole ms and a fraction in [0, 1).
+        const double pa = cubic(S.phase, tau), pb = cubic(S.phase, tau + 1.0);
+        const double xa = cubic(S.delay, tau) - sub, xb = cubic(S.delay, tau + 1.0) - sub;
+        const double ua = ceil(xa), ub = ceil(xb);
+        const int64_t ma = ta - (int64_t)ua, mb = ta + 1 - (int64_t)ub;
+        const double sa = ua - xa, sb = ub - xb;
+        const double n = (double)spe;
+        const int64_t bms = ma + S.base_offset_ms;
+        const double span_ms = (double)(mb - ma) + (sb - sa);
+        if (fabs(span_ms - 1.0) <= 0.03 && bms >= 0) {
+            const int64_t cms = (int64_t)S.code_ms;
+            const int64_t period0 = div_floor64(bms, cms);
+            const double in_period = (double)mod_floor64(bms, cms) + sa;
+            const double epm = (double)S.entries_per_ms;
+            const int64_t num = mod_floor64(mod_floor64(S.if_hz, 1000) * mod_floor64(ta, 1000), 1000);
+            double ifp = (double)num / 1000.0 + (double)S.if_hz * sub / 1000.0;
+            ifp = ifp - floor(ifp);
+            const double phase0 = ifp - pa;
+            const uint64_t k = (uint64_t)period0;
+            r.carrier_phase0 = phase0 - floor(phase0);
+            r.carrier_step = (double)S.if_hz / (n * 1000.0) + (pa - pb) / n;
+            r.entry_phase0 = in_period * epm;
+            r.entries_per_sample = span_ms * epm / n;
+            r.period0 = k;
+            r.table = S.table;
+            r.amp = (float)(S.amp[1] * tau + S.amp[0]);
+            r.lead_bits = S.lead_bits;
+            r.lead_entries = S.lead_bits ? S.lead_entries : 0.0;
+            // The row's masks from the segment's range; a period outside it leaves the row no mask bytes, which the
+            // synthesis kernels report as missing masks (status 4) rather than read past the range.
+            if (k >= S.mask_first && k < S.mask_first + S.mask_count) {
+                r.mask_offset = S.mask_offset + (uint32_t)(k - S.mask_first);
+                r.mask_count = (uint16_t)min<uint64_t>(S.mask_first + S.mask_count - k, 65535u);
+            } else {
+                r.mask_offset = S.mask_offset;
+                r.mask_count = 0;
+            }
+        }
+    }
+    rows[i] = r;
+}
+
+extern "C" int pharos_synth_derive(const pharos_synth_segment *segs, uint32_t block_epochs, uint64_t first_epoch,
+                                   uint32_t epochs, uint32_t channels, int64_t run_start_ms, double run_start_sub_ms,
+                                   uint32_t samples_per_epoch, pharos_synth_row *rows, void *stream) {
+    g_error[0] = 0;
+    if (!rows || (channels && !segs)) { set_error("null argument"); return ST_ARG; }
+    if (block_epochs == 0 || samples_per_epoch == 0) { set_error("block_epochs and samples_per_epoch must be positive"); return ST_ARG; }
+    if (!(run_start_sub_ms >= 0.0 && run_start_sub_ms < 1.0)) { set_error("run_start_sub_ms must be in [0, 1)"); return ST_ARG; }
+    const uint64_t n = (uint64_t)epochs * channels;
+    if (n == 0) return ST_OK;
+    const uint64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffffu) { set_error("too many rows in one call"); return ST_ARG; }
+    hipLaunchKernelGGL(derive_kernel, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, segs, block_epochs,
+                       first_epoch, epochs, channels, run_start_ms, run_start_sub_ms, samples_per_epoch, rows);
+    HIP_TRY(hipGetLastError(), ST_INTERNAL);
+    return ST_OK;
+}
+
 // The batched form's pre-pass: RowPrep of every active row of a job, one thread per row.
 __global__ __launch_bounds__(256) void row_prep_kernel(const pharos_synth_row *rows, uint64_t n, const double *turn,
                                                        RowPrep *out) {

@@ -221,19 +221,46 @@ def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=No
     return result, clips, agc.gain, agc.adjustments()
 
 
-class Job:
-    """One run in a batch: its engine, device rows/masks/output and its gain state."""
+def derive_rows(scenario, first_epoch, epochs, block_epochs=100, device=0, stream=None, segments=None):
+    """Rows on the device from the scenario's per-block segments (pharos_synth_derive): the host computes one
+    segment per channel and block plus the masks; the device evaluates every epoch's row.  Returns (rows, masks,
+    mask_len) as device buffers."""
+    if segments is None:
+        segments = scenario.segments(first_epoch, epochs, block_epochs)
+    segs, masks = segments
+    nch = len(scenario.channels)
+    d_segs = DeviceBuffer.from_array(segs, device)
+    d_masks = DeviceBuffer.from_array(masks, device)
+    d_rows = DeviceBuffer(max(1, epochs * nch) * 64, device)
+    check(_lib.lib().pharos_synth_derive(d_segs.ptr if nch else None, block_epochs, first_epoch, epochs, nch,
+                                         scenario.start_gps_ms, scenario.start_gps_sub_ms, scenario.samples_per_epoch,
+                                         d_rows.ptr, stream.handle if stream else None))
+    if stream:
+        stream.sync()
+    else:
+        check(_lib.lib().pharos_synth_device_sync(device))
+    return d_rows, d_masks, len(masks)
 
-    def __init__(self, scenario, first_epoch=0, epochs=None, device=0, rows=None, masks=None, adjustments_cap=4096):
+
+class Job:
+    """One run in a batch: its engine, device rows/masks/output and its gain state.  Rows come from host rows and
+    masks (rows=, masks=, or the scenario's host stage), or with derive=True from the scenario's segments, evaluated
+    on the device."""
+
+    def __init__(self, scenario, first_epoch=0, epochs=None, device=0, rows=None, masks=None, adjustments_cap=4096,
+                 derive=False, block_epochs=100):
         self.scenario = scenario
         self.first_epoch = first_epoch
         self.epochs = scenario.epochs if epochs is None else epochs
         self.engine = Engine(scenario, device)
-        if rows is None:
-            rows, masks = scenario.window(first_epoch, self.epochs)
-        self.rows = DeviceBuffer.from_array(rows, device)
-        self.masks = DeviceBuffer.from_array(masks, device)
-        self.mask_len = len(masks)
+        if derive:
+            self.rows, self.masks, self.mask_len = derive_rows(scenario, first_epoch, self.epochs, block_epochs, device)
+        else:
+            if rows is None:
+                rows, masks = scenario.window(first_epoch, self.epochs)
+            self.rows = DeviceBuffer.from_array(rows, device)
+            self.masks = DeviceBuffer.from_array(masks, device)
+            self.mask_len = len(masks)
         self.out = DeviceBuffer(self.epochs * self.engine.bytes_per_epoch, device)
         self.agc = _lib.SynthAgc(1.0, 0, 0, 0, 0)
         self.adjust = (_lib.SynthAdjustment * adjustments_cap)()

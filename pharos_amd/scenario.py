@@ -103,6 +103,26 @@ class Scenario:
         return rows.reshape(epochs, n), masks[:max(int(used), 1)].copy()
 
 
+    def segments(self, first_epoch, epochs, block_epochs=100):
+        """Segments [blocks, channels] (SEGMENT_DTYPE: per block of epochs, each channel's phase and delay cubics
+        and amplitude, for the device derive) and the mask bytes they index."""
+        L = _lib.lib()
+        s, chans = self.spec()
+        n = len(self.channels)
+        blocks = -(-epochs // block_epochs)
+        segs = np.zeros(max(1, blocks * n), dtype=_lib.SEGMENT_DTYPE)
+        per_block = int(L.pharos_scenario_mask_bound(C.byref(s), chans, block_epochs)) + 16 * max(n, 1)
+        cap = per_block * blocks + 16
+        masks = np.zeros(cap, dtype=np.uint8)
+        if n == 0:
+            return segs[:0].reshape(blocks, 0), masks[:1]
+        used = L.pharos_scenario_segments(C.byref(s), chans, first_epoch, epochs, block_epochs, segs.ctypes.data,
+                                          masks.ctypes.data, cap)
+        if used < 0:
+            raise _lib.PharosError(int(-used), "scenario segments failed")
+        return segs.reshape(blocks, n), masks[:max(int(used), 1)].copy()
+
+
 class _TableSet:
     def __init__(self, tabs, n):
         self.array = tabs
