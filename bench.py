@@ -12,7 +12,9 @@ adaptive AGC -- rows and masks resident in HBM before the timed region, the 3.69
                one thread, on a bounded sample
   precision    the f32 carrier rotor against FP64 and the Q32.32 code position against f64, every sample of the run
   run_latency  p50/p99 wall time of the timed batched runs
-  serving      p50/p99 latency of a Poisson request stream (100 req/s for 10 s by default; --no-serve skips it)
+  serving      p50/p99 latency of a Poisson request stream (100 req/s for 10 s by default; --no-serve skips it),
+               rows derived on the device from per-block segments; serving_to_host: 20 req/s with each output
+               copied to host memory, latency to its last byte there
 
 --workload c3 instead measures the uneven batch: 256 seeded scenarios (4-48 channels, 4.092-61.44 Msps,
 IQ2-IQ16, durations cut to --c3-epochs-cap epochs) as one batched persistent launch, against per-scenario plain
@@ -438,6 +440,10 @@ def main():
         from pharos_amd import serve
         line["serving"] = serve.measure(duration_s=args.serve_seconds, rate_hz=args.serve_rate,
                                         epochs_cap=args.serve_epochs_cap or None)
+        # Latency to the last byte in host memory, at a rate whose output volume the host link can carry (the
+        # output of 100 req/s averages about 64 GB/s, more than a PCIe x16 link moves).
+        line["serving_to_host"] = serve.measure(duration_s=5.0, rate_hz=20.0, epochs_cap=args.serve_epochs_cap or None,
+                                                to_host=True)
         if "cpu_baseline" in line:
             line["serving"]["over_cpu"] = line["serving"]["channel_samples_per_s"] / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)

@@ -191,14 +191,20 @@ class HostAgc:
         return [(int(a.epoch), float(a.gain), float(a.clip_rate)) for a in self.log[:n]]
 
 
-def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=None, window=100, gain=None):
-    """Per-window plain launches with the host AGC between windows (gain.zig:19-35, engine/run.zig:238-247).
+def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=None, window=100, gain=None,
+              derive=False):
+    """Per-window plain launches with the host AGC between windows (gain.zig:19-35, engine/run.zig:238-247); rows
+    from the host stage, or with derive=True from the scenario's segments on the device.
 
     Returns (bytes, per-epoch clips, the AGC's gain after the run, its adjustments as (epoch, gain, rate))."""
     epochs = scenario.epochs if epochs is None else epochs
     eng = Engine(scenario, device, format_override)
-    rows, masks = scenario.window(first_epoch, epochs)
-    d_rows, d_masks = DeviceBuffer.from_array(rows, device), DeviceBuffer.from_array(masks, device)
+    if derive:
+        d_rows, d_masks, mask_len = derive_rows(scenario, first_epoch, epochs, device=device)
+        masks = np.zeros(mask_len, dtype=np.uint8)
+    else:
+        rows, masks = scenario.window(first_epoch, epochs)
+        d_rows, d_masks = DeviceBuffer.from_array(rows, device), DeviceBuffer.from_array(masks, device)
     out = DeviceBuffer(epochs * eng.bytes_per_epoch, device)
     nch = len(scenario.channels)
     constant = scenario.gain_db is not None or gain is not None

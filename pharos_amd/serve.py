@@ -18,6 +18,7 @@ import bisect
 import collections
 import ctypes as C
 import os
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 
@@ -27,6 +28,8 @@ from . import _lib
 from ._lib import check
 from .engine import Engine, DeviceBuffer, Stream, require_gpu
 
+BLOCK = 100             # epochs per segment (one AGC window)
+HOST_CHUNK = 256 << 20  # bytes per device-to-host copy of a finished output (serve(to_host=True))
 SLICE_MIN = 1000        # epochs of a request per iteration, at least (ten AGC windows) ...
 SLICE_WORK = 8e9        # ... or about this many channel-samples (a few ms of the GPU), in whole AGC windows
 RESERVE = 8 << 30       # HBM left free for the library's own allocations: engines and the batch-state blocks it
@@ -67,12 +70,12 @@ class Request:
         return min(n, self.epochs)
 
     def slice_bytes(self):
-        """Device bytes of one slice's rows and masks (the mask bound of the scenario stage)."""
-        from . import _lib as L
-        import ctypes as C_
+        """Device bytes of one slice's rows, masks and segments (the scenario stage's bounds)."""
         s, chans = self.scenario.spec()
-        masks = int(L.lib().pharos_scenario_mask_bound(C_.byref(s), chans, self.slice_epochs)) + 16
-        return self.slice_epochs * max(1, len(self.scenario.channels)) * 64, masks
+        n = max(1, len(self.scenario.channels))
+        blocks = -(-self.slice_epochs // BLOCK)
+        masks = (int(_lib.lib().pharos_scenario_mask_bound(C.byref(s), chans, BLOCK)) + 16 * n) * blocks + 16
+        return self.slice_epochs * n * 64, masks, blocks * n * _lib.SEGMENT_DTYPE.itemsize
 
     def state_bytes(self):
         """Device bytes of this request's share of a batch's state: the row pre-pass of one slice (when an epoch
@@ -83,9 +86,9 @@ class Request:
         return STATE_SLACK * (prep + 256 + segments * 64)
 
     def buffer_sizes(self):
-        """Bytes of the request's device buffers: its output, then two (rows, masks) pairs."""
-        rows, masks = self.slice_bytes()
-        return [self.out_bytes, rows, masks, rows, masks]
+        """Bytes of the request's device buffers: its output, then two (rows, masks, segments) sets."""
+        rows, masks, segs = self.slice_bytes()
+        return [self.out_bytes, rows, masks, segs, rows, masks, segs]
 
     def gain_mode(self):
         return 1 if self.scenario.gain_db is not None else 0
@@ -100,11 +103,12 @@ class Request:
 
 
 def _prepare(req, first):
-    """Host rows and masks of the slice starting at `first` (runs in a worker thread; the C stage releases the GIL).
-    A request's first slice is SLICE_MIN epochs, so it starts soon; the later ones carry about SLICE_WORK."""
+    """The host's part of the slice starting at `first` (runs in a worker thread; the C stage releases the GIL):
+    one segment per channel and 100-epoch block and the symbol masks; the rows are derived on the device.  A
+    request's first slice is SLICE_MIN epochs, so it starts soon; the later ones carry about SLICE_WORK."""
     n = min(SLICE_MIN if first == 0 else req.slice_epochs, req.epochs - first)
-    rows, masks = req.scenario.window(first, n)
-    return first, n, rows, masks
+    segs, masks = req.scenario.segments(first, n, BLOCK)
+    return first, n, segs, masks
 
 
 def _arrive(req, device):
@@ -115,15 +119,16 @@ def _arrive(req, device):
 
 
 def _copy_into(buffers, prepared):
-    """Copies a prepared slice into a request's row/mask buffers (worker thread)."""
-    first, n, rows, masks = prepared
-    d_rows, d_masks = buffers
-    if rows.nbytes > d_rows.nbytes or masks.nbytes > d_masks.nbytes:
+    """Copies a prepared slice's segments and masks into a request's buffers (worker thread); its rows are derived
+    into the set's row buffer on the loop's stream just before the batch that reads them."""
+    first, n, segs, masks = prepared
+    d_rows, d_masks, d_segs = buffers
+    if segs.nbytes > d_segs.nbytes or masks.nbytes > d_masks.nbytes:
         raise RuntimeError("slice larger than its buffers")
-    if rows.nbytes:
-        check(_lib.lib().pharos_synth_memcpy(d_rows.ptr, rows.ctypes.data, rows.nbytes, 0))
+    if segs.nbytes:
+        check(_lib.lib().pharos_synth_memcpy(d_segs.ptr, segs.ctypes.data, segs.nbytes, 0))
     check(_lib.lib().pharos_synth_memcpy(d_masks.ptr, masks.ctypes.data, masks.nbytes, 0))
-    return first, n, d_rows, d_masks, len(masks)
+    return first, n, d_rows, d_masks, len(masks), d_segs
 
 
 def _next_slice(req, first, buffers):
@@ -255,6 +260,12 @@ class Server:
         self.pool = ThreadPoolExecutor(max_workers=threads)
         self.arrivals = ThreadPoolExecutor(max_workers=max(1, threads // 4))
         self.stream = Stream(device)
+        # Copies of finished outputs to host memory (serve(to_host=True)): two copier threads, each with a pinned
+        # staging chunk; requests whose copy ended come back through _copied for their device memory to be freed.
+        self.copier = ThreadPoolExecutor(max_workers=2)
+        self._staging = threading.local()
+        self._pinned = []
+        self._copied = collections.deque()
         free, total = C.c_uint64(), C.c_uint64()
         check(_lib.lib().pharos_synth_mem_info(device, C.byref(free), C.byref(total)))
         self.arena = DeviceArena(device, max(0, int(free.value) - reserve))
@@ -272,8 +283,8 @@ class Server:
             self.arena.unhold(req.state_bytes())
             return False
         req.out = got[0]
-        # Two row/mask buffers per request: the next slice is written into one while the batch reads the other.
-        req.buffers = [(got[1], got[2]), (got[3], got[4])]
+        # Two row/mask/segment sets per request: the next slice is written into one while the batch reads the other.
+        req.buffers = [(got[1], got[2], got[3]), (got[4], got[5], got[6])]
         req.flip = 0
         req.admitted = time.perf_counter()
         return True
@@ -296,12 +307,48 @@ class Server:
             req.out = None
             req.engine.close()
 
+    def _to_host(self, req):
+        """Copies a finished request's output to host memory in chunks through a pinned staging buffer (copier
+        thread), stamps when its last byte arrived, and hands the request back for its device output to be freed."""
+        st = self._staging
+        if not getattr(st, "ptr", None):
+            p = C.c_void_p()
+            check(_lib.lib().pharos_synth_host_alloc(HOST_CHUNK, C.byref(p)))
+            st.ptr = p.value
+            self._pinned.append(p.value)
+        for off in range(0, req.out_bytes, HOST_CHUNK):
+            n = min(HOST_CHUNK, req.out_bytes - off)
+            check(_lib.lib().pharos_synth_memcpy(st.ptr, req.out.at(off), n, 1))
+        req.host_done = time.perf_counter()
+        self._copied.append(req)
+
+    def _release_copied(self):
+        while self._copied:
+            req = self._copied.popleft()
+            req.out.free()
+            req.out = None
+            req.engine.close()
+
     def close(self):
-        """Returns the arena's device memory (outputs kept by serve(keep_outputs=True) included)."""
+        """Returns the arena's device memory (outputs kept by serve(keep_outputs=True) included) and the pinned
+        staging chunks."""
+        self.copier.shutdown(wait=True)
+        self._release_copied()
         self.arena.close()
+        for p in self._pinned:
+            _lib.lib().pharos_synth_host_free(p)
+        self._pinned = []
+
+    def _derive(self, req):
+        """Queues the device derive of the request's staged slice on the loop's stream (ahead of its batch)."""
+        first, n, d_rows, d_masks, mask_len, d_segs = req.staged
+        sc = req.scenario
+        check(_lib.lib().pharos_synth_derive(d_segs.ptr, BLOCK, first, n, len(sc.channels), sc.start_gps_ms,
+                                             sc.start_gps_sub_ms, sc.samples_per_epoch, d_rows.ptr,
+                                             self.stream.handle))
 
     def _job(self, req):
-        first, n, d_rows, d_masks, mask_len = req.staged
+        first, n, d_rows, d_masks, mask_len, _ = req.staged
         req.current = (first, n)
         prior = min(req.agc.adjustments, len(req.adjust))
         adj = C.cast(C.c_void_p(C.addressof(req.adjust) + prior * C.sizeof(_lib.SynthAdjustment)),
@@ -312,15 +359,20 @@ class Server:
                              mask_len, req.out.at(first * req.scenario.bytes_per_epoch), req.gain_mode(),
                              req.constant_gain(), C.pointer(req.agc), adj, cap, 0, C.pointer(req.slice_clip))
 
-    def serve(self, requests, on_complete=None, keep_outputs=False):
-        """Runs `requests` (arrival times relative to the call) to completion; returns them with .completed set."""
+    def serve(self, requests, on_complete=None, keep_outputs=False, to_host=False):
+        """Runs `requests` (arrival times relative to the call) to completion; returns them with .completed set (the
+        end of the batch that wrote their last epoch to device memory).  With to_host, each finished output is also
+        copied to host memory by copier threads while the loop goes on, and .host_done is when its last byte
+        arrived there."""
         L = _lib.lib()
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting, active, finished = collections.deque(), [], []
+        copies = []
         t0 = time.perf_counter()
         for r in requests:
             r.arrival_abs = t0 + r.arrival
         while pending or waiting or active:
+            self._release_copied()
             now = time.perf_counter()
             while pending and pending[0].arrival_abs <= now:
                 req = pending.popleft()
@@ -348,7 +400,9 @@ class Server:
                 if pending and not waiting:
                     time.sleep(max(0.0, min(0.002, pending[0].arrival_abs - time.perf_counter())))
                 continue
-            # One persistent batch over every active request's staged slice.
+            # One persistent batch over every active request's staged slice, after the derives of their rows.
+            for r in active:
+                self._derive(r)
             jobs = (_lib.SynthJob * len(active))(*[self._job(r) for r in active])
             h = C.c_void_p()
             check(L.pharos_synth_batch_submit(jobs, len(active), C.byref(self.tuning), self.stream.handle, C.byref(h)))
@@ -376,10 +430,15 @@ class Server:
                     finished.append(r)
                     if on_complete:
                         on_complete(r)
-                    self._retire(r, keep_outputs)
+                    self._retire(r, keep_outputs or to_host)
+                    if to_host:
+                        copies.append(self.copier.submit(self._to_host, r))
                 else:
                     still.append(r)
             active = still
+        for f in copies:
+            f.result()
+        self._release_copied()
         return finished
 
 
@@ -403,18 +462,26 @@ def make_requests(rate_hz, duration_s, seed=1, max_out_bytes=2 << 30, epochs_cap
     return reqs
 
 
-def measure(duration_s=10.0, rate_hz=20.0, seed=1, epochs_cap=None):
-    """p50/p99 latency and aggregate channel-samples/s of a Poisson request stream."""
+def measure(duration_s=10.0, rate_hz=20.0, seed=1, epochs_cap=None, to_host=False):
+    """p50/p99 latency and aggregate channel-samples/s of a Poisson request stream: latency from arrival to the
+    last byte in device memory and, with to_host, to the last byte in host memory."""
     reqs = make_requests(rate_hz, duration_s, seed, epochs_cap=epochs_cap)
     server = Server()
     t0 = time.perf_counter()
-    done = server.serve(reqs)
+    done = server.serve(reqs, to_host=to_host)
     makespan = time.perf_counter() - t0
     server.close()
     lat = np.array([r.completed - r.arrival_abs for r in done]) * 1e3
     cs = sum(r.scenario.channel_samples for r in done)
+    host = {}
+    if to_host:
+        hl = np.array([r.host_done - r.arrival_abs for r in done]) * 1e3
+        host = {"host_p50_ms": float(np.percentile(hl, 50)), "host_p99_ms": float(np.percentile(hl, 99)),
+                "host_bytes": int(sum(r.out_bytes for r in done))}
     return {"requests": len(done), "rate_hz": rate_hz, "arrival_window_s": duration_s, "makespan_s": makespan,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)), **host,
+            "latency_to": "last byte in device memory (p50/p99); last byte in host memory (host_p50/p99)",
+            "rows": "derived on the device from per-block segments (pharos_synth_derive)",
             "mean_ms": float(lat.mean()), "channel_samples": int(cs), "channel_samples_per_s": cs / makespan,
             "iterations": server.iterations, "mean_batch_jobs": float(np.mean(server.batch_sizes)),
             "host_phase_s": {k: round(v, 3) for k, v in server.phase_s.items()},

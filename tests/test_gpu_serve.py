@@ -25,7 +25,7 @@ def test_requests_in_the_mix_equal_alone(monkeypatch):
     assert len(done) == len(reqs)
     assert max(server.batch_sizes) >= 2, "requests must overlap on the device"
     for r in done:
-        pb, pc, pgain, padj = run_plain(r.scenario)
+        pb, pc, pgain, padj = run_plain(r.scenario, derive=True)  # the loop derives rows on the device
         assert np.array_equal(outs[r.id], pb), r.scenario.name
         assert r.clipped_total == int(pc.sum()), r.scenario.name
         assert r.adjustments() == padj and r.agc.gain == pgain, r.scenario.name
