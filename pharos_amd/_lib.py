@@ -75,6 +75,45 @@ class ScenarioSpec(C.Structure):
                 ("leap_seconds", C.c_int32), ("reserved", C.c_uint32), ("start_gps_sub_ms", C.c_double)]
 
 
+# ----------------------------------------------------------------------------- pharos_run.h records
+class RunOptions(C.Structure):
+    _fields_ = [("workers", C.c_uint32), ("seed", C.c_uint64), ("noise", C.c_uint32), ("gain", C.c_uint32),
+                ("gain_db", C.c_double)]
+
+
+class RunManifestFields(C.Structure):
+    _fields_ = [("start_time_iso", C.c_char_p), ("run_id", C.c_char_p), ("output_path", C.c_char_p)]
+
+
+class RunInfo(C.Structure):
+    _fields_ = [("format_bits", C.c_uint32), ("sample_rate_hz", C.c_uint64), ("center_hz", C.c_int64),
+                ("samples_per_epoch", C.c_uint32), ("workers", C.c_uint32), ("bytes_per_epoch", C.c_uint64),
+                ("total_epochs", C.c_uint64), ("start_gps_ms", C.c_int64), ("start_gps_sub_ms", C.c_double),
+                ("leap_change_gps_ms", C.c_int64), ("leap_seconds", C.c_int32), ("leap_seconds_after", C.c_int32)]
+
+
+class RunCommand(C.Structure):
+    _fields_ = [("apply_at_epoch", C.c_uint64), ("value", C.c_double), ("kind", C.c_uint32), ("system", C.c_uint32),
+                ("svid", C.c_uint32), ("unit", C.c_uint32), ("use_default", C.c_uint32)]
+
+
+class RunApplied(C.Structure):
+    _fields_ = [("epoch", C.c_uint64), ("command", RunCommand)]
+
+
+class RunBlock(C.Structure):
+    _fields_ = [("epoch_first", C.c_uint64), ("epochs", C.c_uint32), ("bytes", C.c_size_t), ("clipped", C.c_uint64),
+                ("applied", C.POINTER(RunApplied)), ("applied_count", C.c_size_t)]
+
+
+class RunEvent(C.Structure):
+    _fields_ = [("epoch", C.c_uint64), ("a", C.c_double), ("b", C.c_double), ("c", C.c_double), ("kind", C.c_uint32),
+                ("code", C.c_uint32), ("system", C.c_uint32), ("svid", C.c_uint32), ("signal", C.c_uint32)]
+
+
+RUN_STATUS = {"ok": 0, "end": 1, "arg": 2, "nomem": 3, "internal": 4, "input": 5, "io": 6, "bundle": 7, "busy": 8}
+
+
 # pharos_synth_row as a numpy record (64 bytes, include/pharos_synth.h)
 ROW_DTYPE = np.dtype([("carrier_phase0", "<f8"), ("carrier_step", "<f8"), ("entry_phase0", "<f8"),
                       ("entries_per_sample", "<f8"), ("lead_entries", "<f8"), ("period0", "<u8"), ("table", "<u4"),
@@ -158,6 +197,21 @@ def lib():
         "pharos_scenario_endpoint": (None, [P(ScenarioSpec), P(ChannelSpec), I64, P(C.c_double), P(I64), P(C.c_double)]),
         "pharos_derive": (C.c_int, [P(ScenarioSpec), P(ChannelSpec), U32, U64, VP]),
         "pharos_scenario_nav_bit": (C.c_int, [P(ChannelSpec), I64, U32]),
+        "pharos_run_abi_version": (U32, []),
+        "pharos_run_source_digest": (U64, []),
+        "pharos_run_last_error": (C.c_char_p, []),
+        "pharos_run_create": (C.c_int, [VP, C.c_size_t, P(RunOptions), P(VP)]),
+        "pharos_run_info_get": (C.c_int, [VP, P(RunInfo)]),
+        "pharos_run_pull": (C.c_int, [VP, VP, C.c_size_t, U32, P(RunBlock)]),
+        "pharos_run_seek": (C.c_int, [VP, U64]),
+        "pharos_run_submit": (C.c_int, [VP, P(RunCommand)]),
+        "pharos_run_next_event": (C.c_int, [VP, P(RunEvent)]),
+        "pharos_run_events_dropped": (U32, [VP]),
+        "pharos_run_manifest_json": (C.c_int, [VP, P(RunManifestFields), VP, C.c_size_t, P(C.c_size_t)]),
+        "pharos_run_tag_text": (C.c_int, [VP, VP, C.c_size_t, P(C.c_size_t)]),
+        "pharos_run_to_file": (C.c_int, [VP, C.c_char_p, P(U64)]),
+        "pharos_run_error": (C.c_char_p, [VP]),
+        "pharos_run_destroy": (None, [VP]),
         "pharos_scenario_segments": (I64, [P(ScenarioSpec), P(ChannelSpec), U64, U32, U32, VP, VP, U64]),
         "pharos_synth_derive": (C.c_int, [VP, U32, U64, U32, U32, I64, C.c_double, U32, VP, VP]),
     }

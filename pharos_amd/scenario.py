@@ -103,6 +103,14 @@ class Scenario:
         return rows.reshape(epochs, n), masks[:max(int(used), 1)].copy()
 
 
+    def bundle(self, center_hz=0):
+        """The scenario as a pharos_run.h scenario bundle (PHSC, include/pharos_run.h)."""
+        s, chans = self.spec()
+        head = b"PHSC" + (1).to_bytes(4, "little") + bytes(s)
+        head += self.format_id.to_bytes(4, "little") + bytes(4) + int(center_hz).to_bytes(8, "little", signed=True)
+        head += int(self.epochs).to_bytes(8, "little")
+        return head + b"".join(bytes(chans[i]) for i in range(len(self.channels)))
+
     def segments(self, first_epoch, epochs, block_epochs=100):
         """Segments [blocks, channels] (SEGMENT_DTYPE: per block of epochs, each channel's phase and delay cubics
         and amplitude, for the device derive) and the mask bytes they index."""
