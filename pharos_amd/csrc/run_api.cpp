@@ -11,6 +11,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <future>
 #include <new>
 #include <string>
 #include <vector>
@@ -103,8 +104,16 @@ struct pharos_run {
     void *stream = nullptr;
     void *d_segs = nullptr, *d_masks = nullptr, *d_rows = nullptr, *d_out = nullptr;
     uint64_t masks_cap = 0, bpe = 0;
-    std::vector<pharos_synth_segment> segs;
-    std::vector<uint8_t> masks;
+    // The host stage's output for one block: its segments and masks, and the channel levels they were made with.
+    struct Prepared {
+        uint64_t bstart = UINT64_MAX;
+        std::vector<double> cn0;
+        std::vector<pharos_synth_segment> segs;
+        std::vector<uint8_t> masks;
+        int64_t used = -1;
+    };
+    Prepared current;
+    std::future<Prepared> next;  // the following block, prepared while the GPU runs this one
     std::vector<uint64_t> clips;
     // State
     uint64_t epoch = 0, entry_epoch = 0;
@@ -139,6 +148,7 @@ namespace {
 
 void release(pharos_run *r) {
     if (!r) return;
+    if (r->next.valid()) r->next.wait();
     if (r->stream) pharos_synth_stream_sync(r->stream);
     for (void *p : {r->d_segs, r->d_masks, r->d_rows, r->d_out})
         if (p) pharos_synth_free(p);
@@ -223,24 +233,61 @@ uint32_t window_at(const pharos_run *r, uint64_t first, uint64_t limit) {
     return (uint32_t)n;
 }
 
+// The host stage for the 100-epoch block starting at bstart with the given channel levels (any thread).
+pharos_run::Prepared prepare(const pharos_run *r, uint64_t bstart, std::vector<double> cn0) {
+    pharos_run::Prepared p;
+    const uint32_t nch = (uint32_t)r->channels.size();
+    std::vector<pharos_channel_spec> now = r->channels;
+    for (uint32_t c = 0; c < nch; c++) now[c].cn0_dbhz = cn0[c];
+    const uint32_t blen = (uint32_t)std::min<uint64_t>(WINDOW, r->total_epochs - bstart);
+    p.bstart = bstart;
+    p.cn0 = std::move(cn0);
+    p.segs.resize(std::max<uint32_t>(nch, 1));
+    p.masks.resize(r->masks_cap);
+    p.used = pharos_scenario_segments(&r->spec, now.data(), bstart, blen, blen, p.segs.data(), p.masks.data(),
+                                      r->masks_cap);
+    return p;
+}
+
+std::vector<double> levels(const pharos_run *r) {
+    std::vector<double> v(r->channels.size());
+    for (size_t c = 0; c < v.size(); c++) v[c] = r->cn0(c);
+    return v;
+}
+
 // The window's samples on the GPU: the segments of the 100-epoch block holding the window (fitted over the whole
 // block, so a row does not depend on where windows split it, at a command or a seek) with the channels' current
 // C/N0, the rows derived on the device from the block's start, the plain window, and the bytes copied into `out`.
-// Returns a status and the clip total.
+// While the GPU runs the window, a worker prepares the next block's segments and masks at the same levels (the
+// double-buffered rows of engine/run.zig:382-407); a command that changes a level before that block is used makes
+// it prepared again.  Returns a status and the clip total.
 int synthesise(pharos_run *r, uint64_t first, uint32_t count, uint8_t *out, uint64_t *clipped) {
     const uint32_t nch = (uint32_t)r->channels.size();
-    std::vector<pharos_channel_spec> now = r->channels;
-    for (uint32_t c = 0; c < nch; c++) now[c].cn0_dbhz = r->cn0(c);
     const uint64_t bstart = first - first % WINDOW;
     const uint32_t blen = (uint32_t)std::min<uint64_t>(WINDOW, r->total_epochs - bstart);
     const uint32_t skip = (uint32_t)(first - bstart);
-    const int64_t used = pharos_scenario_segments(&r->spec, now.data(), bstart, blen, blen, r->segs.data(),
-                                                  r->masks.data(), r->masks_cap);
+    std::vector<double> now = levels(r);
+    if (r->current.bstart != bstart || r->current.cn0 != now) {
+        if (r->next.valid()) {
+            pharos_run::Prepared p = r->next.get();
+            r->current = p.bstart == bstart && p.cn0 == now ? std::move(p) : prepare(r, bstart, now);
+        } else {
+            r->current = prepare(r, bstart, now);
+        }
+    }
+    const int64_t used = r->current.used;
     if (used < 0) { r->err.set("the host stage could not prepare epochs %llu-%llu", (unsigned long long)first,
                                (unsigned long long)(first + count - 1)); return PHAROS_RUN_E_INTERNAL; }
     int st = 0;
-    if (nch) st = pharos_synth_memcpy(r->d_segs, r->segs.data(), sizeof(pharos_synth_segment) * nch, 0);
-    if (!st) st = pharos_synth_memcpy(r->d_masks, r->masks.data(), (uint64_t)std::max<int64_t>(used, 1), 0);
+    if (nch) st = pharos_synth_memcpy(r->d_segs, r->current.segs.data(), sizeof(pharos_synth_segment) * nch, 0);
+    if (!st) st = pharos_synth_memcpy(r->d_masks, r->current.masks.data(), (uint64_t)std::max<int64_t>(used, 1), 0);
+    const uint64_t following = bstart + WINDOW;
+    if (!r->next.valid() && following < r->total_epochs) {
+        try {
+            r->next = std::async(std::launch::async, prepare, r, following, now);
+        } catch (...) {  // no thread to be had: the next block is prepared when it is reached
+        }
+    }
     if (!st && nch)
         st = pharos_synth_derive((const pharos_synth_segment *)r->d_segs, blen, bstart, skip + count, nch,
                                  r->spec.start_gps_ms, r->spec.start_gps_sub_ms, r->spec.samples_per_epoch,
@@ -396,8 +443,6 @@ int pharos_run_create(const uint8_t *bundle, size_t len, const pharos_run_option
     }
     r->bpe = pharos_synth_bytes_per_epoch(r->engine);
     r->masks_cap = pharos_scenario_mask_bound(&r->spec, r->channels.data(), WINDOW) + 64ull * std::max<uint32_t>(nch, 1);
-    r->segs.resize(std::max<uint32_t>(nch, 1));
-    r->masks.resize(r->masks_cap);
     r->clips.resize(WINDOW);
     r->cn0_default.resize(nch);
     r->cn0_commanded.resize(nch);
