@@ -25,6 +25,9 @@ HIP_FLAGS = [
     # Branches on wave-uniform values (a channel's kind) stay plain scalar branches instead of structurised flow
     # blocks, so each channel's loop body updates the channel sums in place: no copies of the 64 sums per channel.
     "-mllvm", "-structurizecfg-skip-uniform-regions=true",
+    # Single-lane atomics stay plain: the optimizer's lane scan reads an atomic's result right after it, so the wave
+    # would wait out the work queue's atomic where the kernel means it to return during the channel loop.
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
     "-Wall", "-Wno-unused-parameter", "-Wno-unused-function",
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-parameter",

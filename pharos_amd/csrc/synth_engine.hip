@@ -91,7 +91,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngin
     A.chunk_cap = 0;
     A.prep = prep ? prep + (size_t)e * channels : nullptr;
     // The gain is the argument, or a device word another kernel of the same stream wrote (the graph form).
-    uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain_dev ? *gmem(gain_dev) : gain; }, []() {}, err);
+    uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain_dev ? *gmem(gain_dev) : gain; }, []() {}, []() {}, err);
     if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
 }
 
@@ -364,8 +364,11 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
     // q[0..1]: the item, q[2]: its segment, q[3]: the gain broadcast.
     typedef volatile __attribute__((address_space(3))) uint32_t lds_word;
     lds_word *q = (lds_word *)(uintptr_t)lds_address(smem + S.queue_words_offset);
-    uint32_t hint = 0;  // thread 0's segment cursor
-    if (threadIdx.x == 0) {
+    // The work queue is the last thread's: it takes the next item once a tile's channel loop starts and resolves its
+    // segment once the sums are done, so the queue's atomic and the segment search stay off the other threads' path.
+    constexpr uint32_t QT = THREADS - 1;
+    uint32_t hint = 0;  // the queue thread's segment cursor
+    if (threadIdx.x == QT) {
         const unsigned long long item = atomicAdd(S.queue, 1ull);
         q[0] = (uint32_t)item;
         q[1] = (uint32_t)(item >> 32);
@@ -376,22 +379,21 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
     }
     for (;;) {
         __syncthreads();
-        const unsigned long long item = (unsigned long long)q[0] | ((unsigned long long)q[1] << 32);
+        const unsigned long long item = (unsigned long long)__builtin_amdgcn_readfirstlane(q[0]) |
+                                        ((unsigned long long)__builtin_amdgcn_readfirstlane(q[1]) << 32);
         if (item >= S.total_items) break;
-        const uint32_t s = q[2];
-        // The next item is taken now and resolved while this one computes (AfterSums below).
+        const uint32_t s = __builtin_amdgcn_readfirstlane(q[2]);
         unsigned long long next_item = 0;
-        if (threadIdx.x == 0) next_item = atomicAdd(S.queue, 1ull);
         DevSegment seg;
         {
-            const auto &g = gmem(S.segs)[s];
+            const auto &g = cmem(S.segs)[s];
             seg.first_item = g.first_item;
             seg.job = g.job;
             seg.epoch_off = g.epoch_off;
             seg.epochs = g.epochs;
             seg.tiles = g.tiles;
         }
-        const auto &J = gmem(S.jobs)[seg.job];
+        const auto &J = cmem(S.jobs)[seg.job];
         const uint64_t local = item - seg.first_item;
         const uint32_t e_off = seg.epoch_off + (uint32_t)(local / seg.tiles);
         TileArgs A;
@@ -402,17 +404,21 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
         A.channels = J.channels;
         A.tile = (uint32_t)(local % seg.tiles);
         A.epoch = J.first_epoch + e_off;
-        A.out_epoch = J.out + (size_t)e_off * J.engine->bpe;
+        A.out_epoch = J.out + (size_t)e_off * cmem(J.engine)->bpe;
         A.chunk_cap = S.chunk_cap;
         A.prep = J.prep ? J.prep + (size_t)e_off * J.channels : nullptr;
+        const bool adaptive = J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE;
         auto gain_fn = [&]() -> float {
-            if (J.gain_mode == PHAROS_SYNTH_GAIN_CONSTANT) return J.constant_gain;
+            if (!adaptive) return J.constant_gain;
             if (threadIdx.x == 0) q[3] = wait_gain(S, s);
             __syncthreads();
             return __uint_as_float(q[3]);
         };
+        auto before_loop = [&]() {
+            if (threadIdx.x == QT) next_item = atomicAdd(S.queue, 1ull);
+        };
         auto after_sums = [&]() {
-            if (threadIdx.x == 0) {
+            if (threadIdx.x == QT) {
                 q[0] = (uint32_t)next_item;
                 q[1] = (uint32_t)(next_item >> 32);
                 if (next_item < S.total_items) {
@@ -421,15 +427,19 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
                 }
             }
         };
-        const uint32_t clips = synth_tile<F>(A, smem, gain_fn, after_sums, S.err);
+        const uint32_t clips = synth_tile<F>(A, smem, gain_fn, before_loop, after_sums, S.err);
         if (threadIdx.x == 0) {
-            // One atomic counts the tile and its clips; the last tile of the segment gets the segment's total.
+            // One atomic counts the tile and its clips.  Under the adaptive gain the last tile of the segment gets
+            // the segment's total and runs the AGC step; under a constant gain nothing waits for the count.
             const unsigned long long add = (1ull << COUNT_SHIFT) | clips;
-            const unsigned long long now =
-                __hip_atomic_fetch_add(&S.seg_count[s], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + add;
-            if ((now >> COUNT_SHIFT) == (unsigned long long)seg.epochs * seg.tiles &&
-                J.gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE)
-                agc_account(S, J.first_epoch, gmem(J.engine)->spe, seg.job, seg, s, now & CLIP_MASK);
+            if (adaptive) {
+                const unsigned long long now =
+                    __hip_atomic_fetch_add(&S.seg_count[s], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + add;
+                if ((now >> COUNT_SHIFT) == (unsigned long long)seg.epochs * seg.tiles)
+                    agc_account(S, J.first_epoch, cmem(J.engine)->spe, seg.job, seg, s, now & CLIP_MASK);
+            } else {
+                __hip_atomic_fetch_add(&S.seg_count[s], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
 }

@@ -16,6 +16,10 @@ namespace pharos {
 template <class T> using gptr = const __attribute__((address_space(1))) T *;
 template <class T> __device__ __forceinline__ gptr<T> gmem(const T *p) { return (gptr<T>)p; }
 template <class T> using gmut = __attribute__((address_space(1))) T *;
+// Read-only views of memory no kernel writes while it runs (descriptors, job and segment tables): uniform loads
+// through them become scalar loads, which do not wait behind the wave's outstanding vector memory operations.
+template <class T> using cptr = const __attribute__((address_space(4))) T *;
+template <class T> __device__ __forceinline__ cptr<T> cmem(const T *p) { return (cptr<T>)p; }
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 typedef double f64x2_t __attribute__((ext_vector_type(2)));
@@ -621,6 +625,23 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
         uint32_t w[NW];
 #pragma unroll
         for (uint32_t q = 0; q < NW; q++) w[q] = 0;
+        // The group's sums, in 16-byte LDS reads when the whole group is in the tile.
+        float sre[G], sim[G];
+        if (g0 + G <= valid) {
+#pragma unroll
+            for (uint32_t h = 0; h < G; h += 4) {
+                const float4 x = *reinterpret_cast<const float4 *>(L.sum_re + g0 + h);
+                const float4 y = *reinterpret_cast<const float4 *>(L.sum_im + g0 + h);
+                sre[h] = x.x; sre[h + 1] = x.y; sre[h + 2] = x.z; sre[h + 3] = x.w;
+                sim[h] = y.x; sim[h + 1] = y.y; sim[h + 2] = y.z; sim[h + 3] = y.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < G; j++) {
+                sre[j] = g0 + j < valid ? L.sum_re[g0 + j] : 0.0f;
+                sim[j] = g0 + j < valid ? L.sum_im[g0 + j] : 0.0f;
+            }
+        }
         float nre[G], nim[G];
         if (noisy && grouped) {
 #pragma unroll
@@ -635,7 +656,7 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
         for (uint32_t j = 0; j < G; j++) {
             const uint32_t i = g0 + j;
             if (i < valid) {
-                float a = L.sum_re[i], c = L.sum_im[i];
+                float a = sre[j], c = sim[j];
                 if (noisy) {
                     float nr, ni;
                     if (grouped) { nr = nre[j]; ni = nim[j]; }
@@ -688,7 +709,7 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
 
 // The engine descriptor by value, read through a global view (the pointer may come from memory).
 __device__ __forceinline__ DevEngine load_engine(const DevEngine *p) {
-    const auto &g = *gmem(p);
+    const auto &g = *cmem(p);
     DevEngine e;
     e.tables = g.tables;
     e.turn = g.turn;
@@ -721,11 +742,15 @@ struct TileArgs {
 };
 
 
-// One work item of an engine whose output format is F (FORMAT_ANY: the engine's own, chosen per tile).  AfterSums is called by every thread when the last chunk's
-// sums are done (the persistent kernel fetches its next item there); GainFn by every thread before quantising (the
-// persistent kernel waits there for the AGC of the previous window).  Returns the clipped components of the tile.
-template <uint32_t F, class GainFn, class AfterSums>
-__device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, AfterSums after_sums, uint32_t *err) {
+// One work item of an engine whose output format is F (FORMAT_ANY: the engine's own, chosen per tile).  BeforeLoop
+// is called by every thread once the first chunk is set up, before its channel loop, which makes no global memory
+// access (the persistent kernel takes its next item there, so the queue's atomic returns while the sums run);
+// AfterSums by every thread when the last chunk's sums are done (the persistent kernel resolves that item there);
+// GainFn by every thread before quantising (the persistent kernel waits there for the AGC of the previous window).
+// Returns the clipped components of the tile.
+template <uint32_t F, class GainFn, class BeforeLoop, class AfterSums>
+__device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, BeforeLoop before_loop,
+                               AfterSums after_sums, uint32_t *err) {
     DevEngine E = load_engine(A.E);
     if (A.chunk_cap && A.chunk_cap < E.chunk) E.chunk = A.chunk_cap;
     // Passes of equal size: as few passes as the pass size allows, the channels split evenly over them.
@@ -775,6 +800,7 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
                                   A.prep ? A.prep + c0 + c : nullptr, E, n0, tile_len, A.masks, A.mask_len, err, j);
         }
         __syncthreads();
+        if (c0 == 0) before_loop();
         // The next chunk's rows travel from global memory while this chunk computes.
         const uint32_t c1 = c0 + E.chunk;
         const uint32_t nc_next = c1 < A.channels ? min(E.chunk, A.channels - c1) : 0u;
@@ -818,7 +844,10 @@ __device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn,
         if (c1 >= A.channels) after_sums();
         __syncthreads();
     }
-    if (A.channels == 0) after_sums();
+    if (A.channels == 0) {
+        before_loop();
+        after_sums();
+    }
 
     // The sums leave the rotor-lane layout for LDS; the epilogue then walks contiguous samples.
     if (block_live) {
