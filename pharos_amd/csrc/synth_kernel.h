@@ -6,6 +6,7 @@
 // docs/design/synthesis.md:58-73,88-90).  Built with -ffp-contract=off; fmaf/fma exactly where the reference
 // writes @mulAdd.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include "synth_device.h"
 
@@ -621,13 +622,15 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
     uint32_t clip = 0;
     // Groups of 4 samples line up with the noise groups when the tile starts on a multiple of 4 absolute samples.
     const bool grouped = (abs0 & 3u) == 0;
-    for (uint32_t g0 = threadIdx.x * G; g0 < valid; g0 += THREADS * G) {
+    // One group; FULL: all G samples lie in the tile, so no per-sample test remains.
+    auto group = [&](uint32_t g0, auto full) {
+        constexpr bool FULL = decltype(full)::value;
         uint32_t w[NW];
 #pragma unroll
         for (uint32_t q = 0; q < NW; q++) w[q] = 0;
         // The group's sums, in 16-byte LDS reads when the whole group is in the tile.
         float sre[G], sim[G];
-        if (g0 + G <= valid) {
+        if constexpr (FULL) {
 #pragma unroll
             for (uint32_t h = 0; h < G; h += 4) {
                 const float4 x = *reinterpret_cast<const float4 *>(L.sum_re + g0 + h);
@@ -642,8 +645,9 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
                 sim[j] = g0 + j < valid ? L.sum_im[g0 + j] : 0.0f;
             }
         }
+        // A partial group takes each sample's noise on its own (equal to the group's: noise_sample).
         float nre[G], nim[G];
-        if (noisy && grouped) {
+        if (FULL && noisy && grouped) {
 #pragma unroll
             for (uint32_t h = 0; h < G; h += 4) {
                 float r4[4], i4[4];
@@ -655,11 +659,11 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
 #pragma unroll
         for (uint32_t j = 0; j < G; j++) {
             const uint32_t i = g0 + j;
-            if (i < valid) {
+            if (FULL || i < valid) {
                 float a = sre[j], c = sim[j];
                 if (noisy) {
                     float nr, ni;
-                    if (grouped) { nr = nre[j]; ni = nim[j]; }
+                    if (FULL && grouped) { nr = nre[j]; ni = nim[j]; }
                     else noise_sample(E.seed, abs0 + i, nr, ni);
                     a = a + nr;
                     c = c + ni;
@@ -689,7 +693,7 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
             }
         }
         const gmut<uint8_t> d = (gmut<uint8_t>)(dst + bytes_for(F, g0));
-        if (g0 + G <= valid && aligned) {
+        if (FULL && aligned) {
             if (UB == 4) *reinterpret_cast<gmut<uint32_t>>(d) = w[0];
             else if (UB == 8) *reinterpret_cast<gmut<u32x2_t>>(d) = u32x2_t{w[0], w[1 % NW]};
             else if (UB == 16) *reinterpret_cast<gmut<u32x4_t>>(d) = u32x4_t{w[0], w[1 % NW], w[2 % NW], w[3 % NW]};
@@ -698,11 +702,15 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
                 reinterpret_cast<gmut<u32x4_t>>(d)[1] = u32x4_t{w[4 % NW], w[5 % NW], w[6 % NW], w[7 % NW]};
             }
         } else {
-            const uint32_t nb = g0 + G <= valid ? UB : bytes_for(F, valid - g0);
+            const uint32_t nb = FULL ? UB : bytes_for(F, valid - g0);
 #pragma unroll
             for (uint32_t q = 0; q < UB; q++)
                 if (q < nb) d[q] = (uint8_t)(w[q >> 2] >> (8 * (q & 3)));
         }
+    };
+    for (uint32_t g0 = threadIdx.x * G; g0 < valid; g0 += THREADS * G) {
+        if (g0 + G <= valid) group(g0, std::true_type{});
+        else group(g0, std::false_type{});
     }
     return clip;
 }
