@@ -56,20 +56,26 @@ void galois_code(uint32_t start, uint32_t taps, std::vector<uint8_t> &out, size_
 }
 
 // Quadratic-residue table of a prime (weil.zig:1-14).
-const std::vector<uint8_t> &residues(uint32_t p) {
-    static std::mutex m;
-    static std::vector<std::pair<uint32_t, std::vector<uint8_t>>> cache;
-    std::lock_guard<std::mutex> g(m);
-    for (auto &c : cache)
-        if (c.first == p) return c.second;
+std::vector<uint8_t> residue_table(uint32_t p) {
     std::vector<uint8_t> t(p, 0);
     for (uint64_t i = 1; i < p; i++) t[(i * i) % p] = 1;
-    cache.emplace_back(p, std::move(t));
-    return cache.back().second;
+    return t;
+}
+
+// The tables of the Weil primes the signals use (L1C 10223, B1C 10243 and 3607, B2a 1021), each built once on
+// first use (thread-safe static initialisation) and read without a lock afterwards.
+const uint8_t *residues(uint32_t p) {
+    switch (p) {
+    case 1021: { static const std::vector<uint8_t> t = residue_table(1021); return t.data(); }
+    case 3607: { static const std::vector<uint8_t> t = residue_table(3607); return t.data(); }
+    case 10223: { static const std::vector<uint8_t> t = residue_table(10223); return t.data(); }
+    case 10243: { static const std::vector<uint8_t> t = residue_table(10243); return t.data(); }
+    default: return nullptr;
+    }
 }
 
 uint8_t weil_chip(uint32_t p, uint32_t k, uint32_t w) {
-    const std::vector<uint8_t> &L = residues(p);
+    const uint8_t *L = residues(p);
     return L[k] ^ L[(k + w) % p];
 }
 

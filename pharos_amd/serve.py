@@ -5,8 +5,9 @@ Requests are runs (scenarios) arriving over time. The loop works in iterations:
    starts, carved by an arena), oldest first.
 2. It submits one persistent batch holding the next slice of every admitted request. A slice is a whole number
    of 100-epoch AGC windows, sized so it carries about SLICE_WORK channel-samples.
-3. While the GPU runs that batch, worker threads derive each request's following slice and copy it into the
-   request's second row/mask buffer.
+3. While the GPU runs that batch, worker threads make each request's following slice (its segments and masks) and
+   copy it into the request's second buffer set, through pinned staging on copy streams of their own, so the
+   copies run beside the batch instead of waiting for it.
 4. It retires the requests that completed.
 
 A request's AGC state (pharos_synth_agc) carries from slice to slice, so its bytes are those of the same run alone,
@@ -18,6 +19,7 @@ import bisect
 import collections
 import ctypes as C
 import os
+import queue
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -36,6 +38,9 @@ RESERVE = 8 << 30       # HBM left free for the library's own allocations: engin
                         # caches between batches (at most 4 GiB of them); each request's share of a batch's state
                         # is held against the arena while it is admitted (Request.state_bytes)
 PREP_BYTES = 272        # the batched form's row pre-pass per row (RowPrep) of a job with several tiles per epoch
+MIN_COPY = 64 << 10     # bytes a copy carries at least: the runtime moves a copy this large with a DMA engine, beside
+                        # a running batch; a small one (4 KiB measured) goes to a copy kernel, which waits for the
+                        # batch to leave the compute units
 STATE_SLACK = 2         # a batch's state block rounds up to a power of two below 256 MiB
 
 
@@ -85,10 +90,29 @@ class Request:
         segments = self.slice_epochs // 100 + 2
         return STATE_SLACK * (prep + 256 + segments * 64)
 
+    def spec(self):
+        """The scenario's ctypes records (built once: slices reuse them)."""
+        if getattr(self, "_spec", None) is None:
+            self._spec = self.scenario.spec()
+        return self._spec
+
+    def mask_cap(self, epochs):
+        """Bytes the masks of a slice of `epochs` epochs can take (the host stage's bound)."""
+        s, chans = self.spec()
+        n = max(1, len(self.scenario.channels))
+        blocks = -(-epochs // BLOCK)
+        return (int(_lib.lib().pharos_scenario_mask_bound(C.byref(s), chans, BLOCK)) + 16 * n) * blocks + 16
+
+    def io_split(self):
+        """Offset of the masks in a slice's segment-and-mask buffer (the segments come first)."""
+        return (self.slice_bytes()[2] + 255) // 256 * 256
+
     def buffer_sizes(self):
-        """Bytes of the request's device buffers: its output, then two (rows, masks, segments) sets."""
-        rows, masks, segs = self.slice_bytes()
-        return [self.out_bytes, rows, masks, segs, rows, masks, segs]
+        """Bytes of the request's device buffers: its output, then two (rows, segments + masks) sets.  A set's
+        segments and masks share one buffer, so one copy of at least MIN_COPY bytes brings both."""
+        rows, masks, _ = self.slice_bytes()
+        io = max(self.io_split() + masks, MIN_COPY)
+        return [self.out_bytes, rows, io, rows, io]
 
     def gain_mode(self):
         return 1 if self.scenario.gain_db is not None else 0
@@ -118,22 +142,12 @@ def _arrive(req, device):
     return _prepare(req, 0)
 
 
-def _copy_into(buffers, prepared):
-    """Copies a prepared slice's segments and masks into a request's buffers (worker thread); its rows are derived
-    into the set's row buffer on the loop's stream just before the batch that reads them."""
-    first, n, segs, masks = prepared
-    d_rows, d_masks, d_segs = buffers
-    if segs.nbytes > d_segs.nbytes or masks.nbytes > d_masks.nbytes:
-        raise RuntimeError("slice larger than its buffers")
-    if segs.nbytes:
-        check(_lib.lib().pharos_synth_memcpy(d_segs.ptr, segs.ctypes.data, segs.nbytes, 0))
-    check(_lib.lib().pharos_synth_memcpy(d_masks.ptr, masks.ctypes.data, masks.nbytes, 0))
-    return first, n, d_rows, d_masks, len(masks), d_segs
+class View:
+    """A device range inside a larger buffer: .ptr and .nbytes, as the loop's buffers have."""
+    __slots__ = ("ptr", "nbytes")
 
-
-def _next_slice(req, first, buffers):
-    """The slice at `first`, derived and copied into `buffers` (one worker task)."""
-    return _copy_into(buffers, _prepare(req, first))
+    def __init__(self, ptr, nbytes):
+        self.ptr, self.nbytes = int(ptr), int(nbytes)
 
 
 class ArenaBuffer:
@@ -260,11 +274,23 @@ class Server:
         self.pool = ThreadPoolExecutor(max_workers=threads)
         self.arrivals = ThreadPoolExecutor(max_workers=max(1, threads // 4))
         self.stream = Stream(device)
+        # Copy streams, made right after the loop's stream so that each gets a hardware queue of its own (streams
+        # share a process's few hardware queues in turn, and a copy queued behind the running batch on a shared
+        # queue would wait for it).  Every worker's slice copies go on one, finished outputs on the other.
+        self.copy_stream = Stream(device)
+        self.host_stream = Stream(device)
         # Copies of finished outputs to host memory (serve(to_host=True)): two copier threads, each with a pinned
         # staging chunk; requests whose copy ended come back through _copied for their device memory to be freed.
         self.copier = ThreadPoolExecutor(max_workers=2)
-        self._staging = threading.local()
+        # Staging records for copies (a pinned buffer and a copy stream each), one per thread that copies.
         self._pinned = []
+        self._pinned_lock = threading.Lock()
+        self._stagings = queue.SimpleQueue()       # slices: the workers and the loop
+        self._host_stagings = queue.SimpleQueue()  # finished outputs: the copier threads
+        for _ in range(threads + 1):
+            self._stagings.put({"ptr": 0, "nbytes": 0, "stream": self.copy_stream})
+        for _ in range(2):
+            self._host_stagings.put({"ptr": 0, "nbytes": 0, "stream": self.host_stream})
         self._copied = collections.deque()
         free, total = C.c_uint64(), C.c_uint64()
         check(_lib.lib().pharos_synth_mem_info(device, C.byref(free), C.byref(total)))
@@ -272,6 +298,82 @@ class Server:
         self.iterations = 0
         self.batch_sizes = []
         self.phase_s = collections.Counter()  # host seconds per loop phase
+
+    def _borrow(self, nbytes, pool=None):
+        """A staging record (pinned host buffer of at least nbytes, and the copy stream) for one task; _give_back
+        returns it.  Copies go through them as asynchronous copies on a copy stream: a synchronous copy waits for
+        the batch running on the device, an asynchronous one from pinned memory runs beside it.  The streams are
+        made when the loop starts, since creating one waits for the device to go idle."""
+        st = (pool or self._stagings).get()
+        st["pool"] = pool or self._stagings
+        if st["nbytes"] < nbytes:
+            p = C.c_void_p()
+            size = max(int(nbytes), st["nbytes"] * 2)
+            check(_lib.lib().pharos_synth_host_alloc(size, C.byref(p)))
+            with self._pinned_lock:
+                self._pinned.append(p.value)
+            st["ptr"], st["nbytes"] = p.value, size
+        return st
+
+    def _give_back(self, st):
+        st["pool"].put(st)
+
+    def _copy_into(self, req, buffers, prepared):
+        """Copies a prepared slice's segments and masks into one of the request's buffer sets (worker thread) in one
+        copy of at least MIN_COPY bytes; its rows are derived into the set's row buffer on the loop's stream just
+        before the batch that reads them."""
+        first, n, segs, masks = prepared
+        d_rows, d_io = buffers
+        split = req.io_split()
+        if segs.nbytes > split or split + masks.nbytes > d_io.nbytes:
+            raise RuntimeError("slice larger than its buffers")
+        count = min(max(split + masks.nbytes, MIN_COPY), d_io.nbytes)
+        st = self._borrow(count)
+        try:
+            staging = np.ctypeslib.as_array((C.c_uint8 * count).from_address(st["ptr"]))
+            staging[:segs.nbytes] = segs.view(np.uint8).reshape(-1)
+            staging[split:split + masks.nbytes] = masks
+            check(_lib.lib().pharos_synth_memcpy_async(d_io.ptr, st["ptr"], count, 0, st["stream"].handle))
+            st["stream"].sync()
+        finally:
+            self._give_back(st)
+        return first, n, d_rows, View(d_io.ptr + split, masks.nbytes), len(masks), View(d_io.ptr, segs.nbytes)
+
+    def _next_slice(self, req, first, buffers):
+        """The slice at `first` (one worker task): the host stage writes its segments and masks straight into a
+        pinned staging buffer, and one copy of at least MIN_COPY bytes takes them to the buffer set.  (Writing into
+        fresh arrays instead made the workers contend for the page faults of the new memory.)"""
+        n = min(SLICE_MIN if first == 0 else req.slice_epochs, req.epochs - first)
+        d_rows, d_io = buffers
+        split = req.io_split()
+        nch = len(req.scenario.channels)
+        blocks = -(-n // BLOCK)
+        cap = req.mask_cap(n)
+        seg_bytes = blocks * nch * _lib.SEGMENT_DTYPE.itemsize
+        if seg_bytes > split or split + cap > d_io.nbytes:
+            raise RuntimeError("slice larger than its buffers")
+        t0 = time.perf_counter()
+        st = self._borrow(max(split + cap, MIN_COPY))
+        try:
+            s, chans = req.spec()
+            used = _lib.lib().pharos_scenario_segments(C.byref(s), chans, first, n, BLOCK, st["ptr"],
+                                                       st["ptr"] + split, cap)
+            if used < 0:
+                raise _lib.PharosError(int(-used), "scenario segments failed")
+            if used == 0:
+                C.memset(st["ptr"] + split, 0, 1)
+            count = min(max(split + int(used), MIN_COPY), d_io.nbytes)
+            t1 = time.perf_counter()
+            check(_lib.lib().pharos_synth_memcpy_async(d_io.ptr, st["ptr"], count, 0, st["stream"].handle))
+            st["stream"].sync()
+            t2 = time.perf_counter()
+        finally:
+            self._give_back(st)
+        with self._pinned_lock:  # per-thread seconds, summed over the workers
+            self.phase_s["slice_host_thread"] += t1 - t0
+            self.phase_s["slice_copy_thread"] += t2 - t1
+        mask_len = max(int(used), 1)
+        return first, n, d_rows, View(d_io.ptr + split, mask_len), mask_len, View(d_io.ptr, seg_bytes)
 
     def _admit(self, req):
         """Places the request's buffers in the arena and holds its share of the batch state; False (and nothing
@@ -284,7 +386,7 @@ class Server:
             return False
         req.out = got[0]
         # Two row/mask/segment sets per request: the next slice is written into one while the batch reads the other.
-        req.buffers = [(got[1], got[2], got[3]), (got[4], got[5], got[6])]
+        req.buffers = [(got[1], got[2]), (got[3], got[4])]
         req.flip = 0
         req.admitted = time.perf_counter()
         return True
@@ -308,17 +410,21 @@ class Server:
             req.engine.close()
 
     def _to_host(self, req):
-        """Copies a finished request's output to host memory in chunks through a pinned staging buffer (copier
-        thread), stamps when its last byte arrived, and hands the request back for its device output to be freed."""
-        st = self._staging
-        if not getattr(st, "ptr", None):
-            p = C.c_void_p()
-            check(_lib.lib().pharos_synth_host_alloc(HOST_CHUNK, C.byref(p)))
-            st.ptr = p.value
-            self._pinned.append(p.value)
-        for off in range(0, req.out_bytes, HOST_CHUNK):
-            n = min(HOST_CHUNK, req.out_bytes - off)
-            check(_lib.lib().pharos_synth_memcpy(st.ptr, req.out.at(off), n, 1))
+        """Copies a finished request's output to host memory in chunks through a pinned staging buffer on a copy
+        stream of its own (copier thread), so the copies run beside the next batches; stamps when its last byte
+        arrived, and hands the request back for its device output to be freed.  A last chunk below MIN_COPY is
+        copied as the output's last MIN_COPY bytes."""
+        st = self._borrow(min(HOST_CHUNK, max(req.out_bytes, 1)), self._host_stagings)
+        try:
+            L = _lib.lib()
+            for off in range(0, req.out_bytes, HOST_CHUNK):
+                n = min(HOST_CHUNK, req.out_bytes - off)
+                if n < MIN_COPY <= req.out_bytes:
+                    off, n = req.out_bytes - MIN_COPY, MIN_COPY
+                check(L.pharos_synth_memcpy_async(st["ptr"], req.out.at(off), n, 1, st["stream"].handle))
+                st["stream"].sync()
+        finally:
+            self._give_back(st)
         req.host_done = time.perf_counter()
         self._copied.append(req)
 
@@ -333,6 +439,8 @@ class Server:
         """Returns the arena's device memory (outputs kept by serve(keep_outputs=True) included) and the pinned
         staging chunks."""
         self.copier.shutdown(wait=True)
+        self.pool.shutdown(wait=True)
+        self.arrivals.shutdown(wait=True)
         self._release_copied()
         self.arena.close()
         for p in self._pinned:
@@ -391,7 +499,7 @@ class Server:
             for req in admitted:
                 prepared = req.arrived.result()
                 req.arrived = None
-                req.staged = _copy_into(self._spare(req), prepared)
+                req.staged = self._copy_into(req, self._spare(req), prepared)
             active.extend(admitted)
             tc = time.perf_counter()
             self.phase_s["admit"] += tb - ta
@@ -413,7 +521,7 @@ class Server:
             # While it runs: the next slice of every request that has one, derived on the host and copied into the
             # request's other row/mask buffer.
             nxt = [(r, sum(r.current)) for r in active if sum(r.current) < r.epochs]
-            futures = [(r, self.pool.submit(_next_slice, r, f, self._spare(r))) for r, f in nxt]
+            futures = [(r, self.pool.submit(self._next_slice, r, f, self._spare(r))) for r, f in nxt]
             for r, f in futures:
                 r.staged = f.result()
             te = time.perf_counter()
