@@ -1156,38 +1156,47 @@ extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t 
     // One launch: a single-format batch runs the kernel instance with only that format's quantiser, a mixed one
     // the instance that picks each job's quantiser per tile.
     uint32_t batch_format = jobs[0].engine->view.format, batch_lds = 0;
-    size_t rounds = 0;
     for (uint32_t j = 0; j < njobs; j++) {
         if (jobs[j].engine->view.format != batch_format) batch_format = FORMAT_ANY;
         batch_lds = std::max(batch_lds, jobs[j].engine->view.lds_bytes);
-        rounds = std::max(rounds, per_job[j].size());
     }
+    // Queue order: window w of a job of W windows at (w + 1/2) / W of the launch, so every job's windows spread over
+    // the whole launch and the jobs end together.  (Window by window across jobs, a job with many short windows ran
+    // its chain of AGC steps alone at the end, on a nearly idle device.)  Each job's windows stay in order, so the
+    // earliest unfinished item never waits on a later one.
+    struct Slot { uint64_t num, den; uint32_t job, win; };
+    std::vector<Slot> order;
+    for (uint32_t j = 0; j < njobs; j++)
+        for (uint32_t w = 0; w < (uint32_t)per_job[j].size(); w++)
+            order.push_back({2ull * w + 1, 2ull * per_job[j].size(), j, w});
+    std::stable_sort(order.begin(), order.end(), [](const Slot &a, const Slot &b) {
+        const unsigned __int128 l = (unsigned __int128)a.num * b.den, r = (unsigned __int128)b.num * a.den;
+        return l != r ? l < r : (a.job != b.job ? a.job < b.job : a.win < b.win);
+    });
     std::vector<DevSegment> segs;
     std::vector<uint32_t> seg_next;
     std::vector<uint32_t> last_seg(njobs, 0xffffffffu);
     h->job_first_seg.assign(njobs, 0xffffffffu);
     h->job_seg_count.assign(njobs, 0);
     uint64_t items = 0;
-    for (size_t r = 0; r < rounds; r++) {
-        for (uint32_t j = 0; j < njobs; j++) {
-            if (r >= per_job[j].size()) continue;
-            const Win &w = per_job[j][r];
-            DevSegment s;
-            s.first_item = items;
-            s.job = j;
-            s.epoch_off = w.off;
-            s.epochs = w.epochs;
-            s.tiles = jobs[j].engine->view.tiles;
-            items += (uint64_t)s.epochs * s.tiles;
-            const uint32_t idx = (uint32_t)segs.size();
-            segs.push_back(s);
-            h->seg_job.push_back(j);
-            seg_next.push_back(0xffffffffu);
-            if (last_seg[j] != 0xffffffffu) seg_next[last_seg[j]] = idx;
-            else h->job_first_seg[j] = idx;
-            last_seg[j] = idx;
-            h->job_seg_count[j]++;
-        }
+    for (const Slot &o : order) {
+        const uint32_t j = o.job;
+        const Win &w = per_job[j][o.win];
+        DevSegment s;
+        s.first_item = items;
+        s.job = j;
+        s.epoch_off = w.off;
+        s.epochs = w.epochs;
+        s.tiles = jobs[j].engine->view.tiles;
+        items += (uint64_t)s.epochs * s.tiles;
+        const uint32_t idx = (uint32_t)segs.size();
+        segs.push_back(s);
+        h->seg_job.push_back(j);
+        seg_next.push_back(0xffffffffu);
+        if (last_seg[j] != 0xffffffffu) seg_next[last_seg[j]] = idx;
+        else h->job_first_seg[j] = idx;
+        last_seg[j] = idx;
+        h->job_seg_count[j]++;
     }
     const uint32_t nsegs = (uint32_t)segs.size();
     h->nsegs = nsegs;
