@@ -100,9 +100,16 @@ class Stream:
     def sync(self):
         check(_lib.lib().pharos_synth_stream_sync(self.handle))
 
+    def close(self):
+        """Destroys the stream now (a process's streams share a few hardware queues: fewer live streams, fewer
+        shared)."""
+        if getattr(self, "handle", None):
+            _lib.lib().pharos_synth_stream_destroy(self.handle)
+            self.handle = None
+
     def __del__(self):
         try:
-            _lib.lib().pharos_synth_stream_destroy(self.handle)
+            self.close()
         except Exception:
             pass
 

@@ -33,7 +33,7 @@ from .engine import Engine, DeviceBuffer, Stream, require_gpu
 BLOCK = 100             # epochs per segment (one AGC window)
 HOST_CHUNK = 256 << 20  # bytes per device-to-host copy of a finished output (serve(to_host=True))
 SLICE_MIN = 1000        # epochs of a request per iteration, at least (ten AGC windows) ...
-SLICE_WORK = 8e9        # ... or about this many channel-samples (a few ms of the GPU), in whole AGC windows
+SLICE_WORK = 16e9       # ... or about this many channel-samples (a few ms of the GPU), in whole AGC windows
 RESERVE = 8 << 30       # HBM left free for the library's own allocations: engines and the batch-state blocks it
                         # caches between batches (at most 4 GiB of them); each request's share of a batch's state
                         # is held against the arena while it is admitted (Request.state_bytes)
@@ -446,6 +446,8 @@ class Server:
         for p in self._pinned:
             _lib.lib().pharos_synth_host_free(p)
         self._pinned = []
+        for st in (self.stream, self.copy_stream, self.host_stream):
+            st.close()
 
     def _derive(self, req):
         """Queues the device derive of the request's staged slice on the loop's stream (ahead of its batch)."""
