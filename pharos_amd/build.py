@@ -12,6 +12,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpharos_amd.so")
+ROOT = os.path.dirname(HERE)
+C_EXAMPLE = os.path.join(ROOT, "tests", "c", "run_pull")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("PHAROS_OFFLOAD_ARCH", "gfx950")
@@ -56,8 +58,16 @@ def build(verbose=True):
             _run(["g++", *CXX_FLAGS, "-c", path, "-o", obj])
             objs.append(obj)
         out_tmp = LIB + ".tmp"
-        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", out_tmp, *objs, "-lpthread"])
+        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", out_tmp, *objs, "-lpthread", "-ldl"])
         os.replace(out_tmp, LIB)
+        # A plain C caller of include/pharos_run.h (tests/c/run_pull.c), linked against the library as a C program
+        # written for the reference's run API would be.
+        c_src = os.path.join(ROOT, "tests", "c", "run_pull.c")
+        if os.path.exists(c_src):
+            exe_tmp = C_EXAMPLE + ".tmp"
+            _run(["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), c_src, "-o",
+                  exe_tmp, "-L", HERE, "-lpharos_amd", f"-Wl,-rpath,{HERE}", f"-Wl,-rpath,{ROCM}/lib"])
+            os.replace(exe_tmp, C_EXAMPLE)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return LIB

@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <dlfcn.h>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -84,8 +85,18 @@ std::vector<uint8_t> g_e1, g_e6;
 std::string g_data_dir;
 std::mutex g_blob_lock;
 
+// The code blobs' folder when the host has not set one: data/ beside this library's own file, where the package
+// keeps them, so a C caller that links the library needs no set-up call.
+std::string library_data_dir() {
+    Dl_info info;
+    if (!dladdr(reinterpret_cast<const void *>(&library_data_dir), &info) || !info.dli_fname) return "data";
+    std::string file = info.dli_fname;
+    const size_t slash = file.rfind('/');
+    return (slash == std::string::npos ? std::string(".") : file.substr(0, slash)) + "/data";
+}
+
 bool load_blob(const char *name, size_t size, std::vector<uint8_t> &into) {
-    std::string path = g_data_dir + "/" + name;
+    std::string path = (g_data_dir.empty() ? library_data_dir() : g_data_dir) + "/" + name;
     FILE *f = fopen(path.c_str(), "rb");
     if (!f) return false;
     into.resize(size);

@@ -6,6 +6,8 @@ oracle's; seek(T) lands on the sequential run's bytes; a signal-power command ch
 apply epoch on, as the same run with that level from the start does."""
 import ctypes as C
 import json
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -158,3 +160,32 @@ def test_pull_rules_to_file_tag_and_manifest(tmp_path):
     assert m["run_id"] == "run-7" and m["samples"]["total_epochs"] == sc.epochs and len(m["channels"]) == len(sc.channels)
     bad_fields = _lib.RunManifestFields(b"\xff\xfe", None, None)
     assert _lib.lib().pharos_run_manifest_json(run.handle, C.byref(bad_fields), big, 65536, C.byref(ln)) == STATUS["arg"]
+
+
+def _weighted(data):
+    """sum of b[i] * (i + 1), mod 2^64 (tests/c/run_pull.c's checksum)."""
+    i = np.arange(1, data.size + 1, dtype=np.uint64)
+    return int(np.sum(data.astype(np.uint64) * i, dtype=np.uint64))
+
+
+def test_a_c_caller_pulls_and_seeks_the_same_bytes(tmp_path):
+    """tests/c/run_pull.c, a plain C program linked against the library, drives the run through pharos_run.h alone:
+    its bytes and clip count equal the Python binding's pull, and its seek lands on the same chunk."""
+    exe = os.path.join(os.path.dirname(__file__), "c", "run_pull")
+    assert os.path.exists(exe), "tests/c/run_pull is built with the library (python -m pharos_amd.build)"
+    sc = small(gain_db=0.0, epochs=330)
+    path = tmp_path / "run.phsc"
+    path.write_bytes(sc.bundle())
+    print("C caller:", exe, sc.name, sc.epochs, "epochs", flush=True)
+    out = subprocess.run([exe, str(path), "37", "123"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = json.loads(out.stdout)
+    run = Run(sc.bundle(), seed=5, gain=1, gain_db=0.0)
+    data = run.pull_all(37)
+    assert got["epochs"] == sc.epochs and got["bytes"] == data.size and got["abi"] == 0x00010000
+    assert got["sum"] == _weighted(data)
+    st, blk, _, _ = Run(sc.bundle(), seed=5, gain=1, gain_db=0.0).pull(sc.epochs)
+    assert st == STATUS["ok"] and got["clipped"] == blk[3]
+    bpe = sc.bytes_per_epoch
+    assert got["seek_first"] == 123 and got["seek_epochs"] == 37
+    assert got["seek_sum"] == _weighted(data[123 * bpe:160 * bpe])

@@ -71,3 +71,54 @@ def test_wyhash_matches_zigs_test_vectors():
 @pytest.mark.skipif(not os.path.isdir("/root/reference/src"), reason="the reference tree is not on this host")
 def test_source_digest_is_the_reference_trees():
     assert source_digest("/root/reference") == _lib.lib().pharos_run_source_digest()
+
+
+C_RUN_PROBE = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "pharos_run.h"
+#include "pharos_synth.h"
+#define F(T, f) printf("[\"%s\", \"%s\", %zu],", #T, #f, offsetof(T, f));
+int main(void) {
+  printf("{\"sizes\": {\"pharos_run_options\": %zu, \"pharos_run_info\": %zu, \"pharos_run_command\": %zu, "
+         "\"pharos_run_applied\": %zu, \"pharos_run_block\": %zu, \"pharos_run_event\": %zu, "
+         "\"pharos_synth_segment\": %zu}, \"fields\": [",
+         sizeof(pharos_run_options), sizeof(pharos_run_info), sizeof(pharos_run_command), sizeof(pharos_run_applied),
+         sizeof(pharos_run_block), sizeof(pharos_run_event), sizeof(pharos_synth_segment));
+  F(pharos_run_options, seed) F(pharos_run_options, gain_db) F(pharos_run_info, sample_rate_hz)
+  F(pharos_run_info, start_gps_sub_ms) F(pharos_run_info, leap_seconds_after) F(pharos_run_command, kind)
+  F(pharos_run_command, use_default) F(pharos_run_block, bytes) F(pharos_run_block, applied_count)
+  F(pharos_run_event, kind) F(pharos_run_event, signal) F(pharos_synth_segment, lead_entries)
+  F(pharos_synth_segment, mask_first) F(pharos_synth_segment, table)
+  printf("[\"end\", \"\", 0]]}\n");
+  return 0;
+}
+"""
+
+
+def test_header_layouts_match_the_c_compiler(tmp_path):
+    """pharos_run.h and the segment record as a C compiler lays them out, against the ctypes binding and the
+    reference's records (c_api/run.zig:25-61, interface/c/command.zig:9-17, c_api/convert.zig:7-19)."""
+    import json
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "probe.c"
+    src.write_text(C_RUN_PROBE)
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = json.loads(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
+    ctypes_of = {"pharos_run_options": _lib.RunOptions, "pharos_run_info": _lib.RunInfo,
+                 "pharos_run_command": _lib.RunCommand, "pharos_run_applied": _lib.RunApplied,
+                 "pharos_run_block": _lib.RunBlock, "pharos_run_event": _lib.RunEvent}
+    for name, size in got["sizes"].items():
+        if name == "pharos_synth_segment":
+            assert size == _lib.SEGMENT_DTYPE.itemsize == 136
+        else:
+            assert size == C.sizeof(ctypes_of[name]), name
+    assert got["sizes"]["pharos_run_block"] == 48
+    for name, field, off in got["fields"][:-1]:
+        if name == "pharos_synth_segment":
+            assert _lib.SEGMENT_DTYPE.fields[field][1] == off, field
+        else:
+            assert getattr(ctypes_of[name], field).offset == off, (name, field)
