@@ -669,7 +669,10 @@ int pharos_run_tag_text(pharos_run *r, uint8_t *buf, size_t cap, size_t *len) {
 }
 
 // pharos_run_to_file (c_api/run.zig:245-259, output/sink.zig:18-47): the rest of the run written from the current
-// epoch, in windows of up to 100 epochs; events raised while writing are dropped (run.zig:292-294).
+// epoch in blocks on the 100-epoch grid (after an off-grid seek the first block stops at the grid), each block
+// written while the next one is pulled into the other half of a double buffer; events raised while writing are
+// dropped (run.zig:292-294).  A failed write stops the pulls and is reported before a generation failure, with the
+// bytes on file in order.
 int pharos_run_to_file(pharos_run *r, const char *path, uint64_t *bytes_written) {
     if (!r) return PHAROS_RUN_E_ARG;
     r->err.clear();
@@ -678,32 +681,57 @@ int pharos_run_to_file(pharos_run *r, const char *path, uint64_t *bytes_written)
     const int fd = open(path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
     if (fd < 0) { r->err.set("cannot write %s: %s", path, strerror(errno)); return PHAROS_RUN_E_IO; }
     const size_t checkpoint = r->len;
-    std::vector<uint8_t> block(WINDOW * r->bpe);
-    uint64_t on_file = 0;
-    int status = PHAROS_RUN_OK;
-    for (;;) {
-        pharos_run_block b;
-        const int st = pharos_run_pull(r, block.data(), block.size(), WINDOW, &b);
-        if (st == PHAROS_RUN_END) break;
-        size_t done = 0;
-        while (done < b.bytes) {
-            const ssize_t w = write(fd, block.data() + done, b.bytes - done);
-            if (w < 0 && errno == EINTR) continue;
-            if (w <= 0) {
-                r->err.set("cannot write %s: %s", path, w < 0 ? strerror(errno) : "no space written");
-                status = PHAROS_RUN_E_IO;
-                break;
-            }
-            done += (size_t)w;
-        }
-        on_file += done;
-        if (status) break;
-        if (st) { status = st; break; }
+    const size_t half = WINDOW * r->bpe;
+    std::vector<uint8_t> pulled;
+    try {
+        pulled.resize(2 * half);
+    } catch (...) {
+        close(fd);
+        r->err.set("no memory for the file's blocks");
+        return PHAROS_RUN_E_NOMEM;
     }
+    uint64_t on_file = 0;
+    int write_errno = 0;  // 0: every write so far complete
+    auto write_block = [fd, &on_file](const uint8_t *p, size_t n) -> int {
+        size_t done = 0;
+        while (done < n) {
+            const ssize_t w = write(fd, p + done, n - done);
+            if (w < 0 && errno == EINTR) continue;
+            if (w <= 0) return w < 0 ? errno : ENOSPC;
+            done += (size_t)w;
+            on_file += (uint64_t)w;
+        }
+        return 0;
+    };
+    std::future<int> writing;
+    int failure = PHAROS_RUN_OK;
+    for (int k = 0;; k ^= 1) {
+        uint8_t *slot = pulled.data() + (size_t)k * half;
+        pharos_run_block b;
+        const uint32_t want = (uint32_t)(WINDOW - r->epoch % WINDOW);
+        const int st = pharos_run_pull(r, slot, half, want, &b);
+        if (writing.valid()) write_errno = writing.get();
+        if (st != PHAROS_RUN_OK && st != PHAROS_RUN_END) failure = st;
+        // The epochs a failing pull produced before its failure are written too (sink.zig:33-36).
+        if (!write_errno && st != PHAROS_RUN_END && b.bytes) {
+            try {
+                writing = std::async(std::launch::async, write_block, slot, (size_t)b.bytes);
+            } catch (...) {  // no thread to be had: write it here
+                write_errno = write_block(slot, (size_t)b.bytes);
+            }
+        }
+        r->len = std::min(r->len, checkpoint);
+        if (failure || st == PHAROS_RUN_END || b.epochs == 0 || write_errno) break;
+    }
+    if (writing.valid()) write_errno = writing.get();
     close(fd);
     r->len = std::min(r->len, checkpoint);
     if (bytes_written) *bytes_written = on_file;
-    return status;
+    if (write_errno) {
+        r->err.set("cannot write %s: %s", path, strerror(write_errno));
+        return PHAROS_RUN_E_IO;
+    }
+    return failure;
 }
 
 const char *pharos_run_error(const pharos_run *r) { return r ? r->err.text : ""; }

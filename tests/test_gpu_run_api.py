@@ -8,6 +8,7 @@ import ctypes as C
 import json
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -189,3 +190,42 @@ def test_a_c_caller_pulls_and_seeks_the_same_bytes(tmp_path):
     bpe = sc.bytes_per_epoch
     assert got["seek_first"] == 123 and got["seek_epochs"] == 37
     assert got["seek_sum"] == _weighted(data[123 * bpe:160 * bpe])
+
+
+RUN_CHILD = r'''
+import ctypes as C, json, os, resource, signal, sys
+sys.path.insert(0, os.getcwd())
+from pharos_amd import _lib
+from pharos_amd.run import Run
+from tests.test_gpu_run_api import small
+signal.signal(signal.SIGXFSZ, signal.SIG_IGN)            # a write past the limit fails with EFBIG instead
+limit = int(sys.argv[2])
+sc = small(gain_db=0.0, epochs=450)
+run = Run(sc.bundle(), seed=sc.seed, gain=1, gain_db=0.0)
+assert run.seek(60) == 0
+resource.setrlimit(resource.RLIMIT_FSIZE, (limit, limit))
+n = C.c_uint64(0)
+st = _lib.lib().pharos_run_to_file(run.handle, os.fsencode(sys.argv[1]), C.byref(n))
+print(json.dumps({"status": st, "bytes_written": n.value, "error": run.error()}), flush=True)
+'''
+
+
+def test_to_file_reports_the_bytes_on_file_after_a_failed_write(tmp_path):
+    """A write that fails part-way (a file size limit) stops the run's file at the bytes written in order, reported
+    with status io (c_api/run.zig:245-259): the first block (epochs 60-99, up to the grid), one whole block, then
+    half of the third."""
+    sc = small(gain_db=0.0, epochs=450)
+    bpe = sc.bytes_per_epoch
+    limit = 40 * bpe + 100 * bpe + 50 * bpe
+    path = tmp_path / "limited.bin"
+    print(f"to_file with a file size limit of {limit} bytes", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", RUN_CHILD, str(path), str(limit)], cwd=root, capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    assert res["status"] == STATUS["io"] and "cannot write" in res["error"], res
+    assert res["bytes_written"] == limit, res
+    ref = Run(sc.bundle(), seed=sc.seed, gain=1, gain_db=0.0).pull_all(100)
+    data = np.fromfile(path, dtype=np.uint8)
+    assert data.size == limit and np.array_equal(data, ref[60 * bpe:60 * bpe + limit])
