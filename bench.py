@@ -15,6 +15,8 @@ adaptive AGC -- rows and masks resident in HBM before the timed region, the 3.69
   serving      p50/p99 latency of a Poisson request stream (100 req/s for 10 s by default; --no-serve skips it),
                rows derived on the device from per-block segments; serving_to_host: 20 req/s with each output
                copied to host memory, latency to its last byte there
+  run_api      the same configuration through include/pharos_run.h: pull into a host buffer in 100-epoch calls
+               and to_file, rates to host memory (--no-run-api skips it)
 
 --workload c3 instead measures the uneven batch: 256 seeded scenarios (4-48 channels, 4.092-61.44 Msps,
 IQ2-IQ16, durations cut to --c3-epochs-cap epochs) as one batched persistent launch, against per-scenario plain
@@ -387,6 +389,45 @@ def bench_c3(args):
     }
 
 
+def bench_run_api(args):
+    """The configuration through pharos_run.h (a C caller's path): the first --run-api-epochs epochs pulled into a
+    host buffer in 100-epoch calls, then written with pharos_run_to_file to a temporary file; the rates include the
+    host stage, the device derive and synthesis, and the copy to host memory (PCIe-bound)."""
+    import ctypes as C
+    import tempfile
+    from pharos_amd import _lib
+    from pharos_amd.run import Run
+    from pharos_amd import scenario as S
+    make = {"c0": S.c0, "c1": S.c1, "c2": S.c2}[args.workload]
+    sc = make(seed=1, epochs=args.run_api_epochs)
+    L = _lib.lib()
+    gain = 1 if sc.gain_db is not None else 0
+    run = Run(sc.bundle(), seed=sc.seed, gain=gain, gain_db=sc.gain_db or 0.0)
+    bpe = run.info().bytes_per_epoch
+    buf = np.empty(100 * bpe, dtype=np.uint8)
+    blk = _lib.RunBlock()
+    L.pharos_run_pull(run.handle, buf.ctypes.data, buf.nbytes, 100, C.byref(blk))   # untimed: first block
+    t = time.perf_counter()
+    n = 0
+    while L.pharos_run_pull(run.handle, buf.ctypes.data, buf.nbytes, 100, C.byref(blk)) == 0:
+        n += blk.epochs
+    pull_s = time.perf_counter() - t
+    per_epoch = sc.samples_per_epoch * len(sc.channels)
+    run = Run(sc.bundle(), seed=sc.seed, gain=gain, gain_db=sc.gain_db or 0.0)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "run.bin")
+        written = C.c_uint64()
+        t = time.perf_counter()
+        st = L.pharos_run_to_file(run.handle, path.encode(), C.byref(written))
+        file_s = time.perf_counter() - t
+    return {"epochs": sc.epochs, "pull": {"epochs": n, "gb_per_s": n * bpe / pull_s / 1e9,
+                                          "channel_samples_per_s": n * per_epoch / pull_s},
+            "to_file": {"status": st, "bytes": written.value, "gb_per_s": written.value / file_s / 1e9,
+                        "channel_samples_per_s": sc.epochs * per_epoch / file_s},
+            "note": "pharos_run.h from a host caller: host stage, device derive and synthesis, copy to pageable host "
+                    "memory; to_file into the page cache of a temporary file"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +448,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window")
     ap.add_argument("--serve-epochs-cap", type=int, default=0, help="longest request in epochs (0: up to 30 s)")
+    ap.add_argument("--no-run-api", action="store_true", help="skip the pharos_run.h pull and to_file rates")
+    ap.add_argument("--run-api-epochs", type=int, default=3000, help="epochs the run API pulls and writes")
     ap.add_argument("--sink", metavar="DIR", default="",
                     help="also time the file sink: the first --sink-epochs epochs of the run written to a file in DIR")
     ap.add_argument("--sink-epochs", type=int, default=6000)
@@ -436,6 +479,8 @@ def main():
     line = bench_c3(args) if args.workload == "c3" else bench_single(args)
     if args.sink and args.workload != "c3":
         line["sink"] = bench_sink(args)
+    if not args.no_run_api and args.workload != "c3":
+        line["run_api"] = bench_run_api(args)
     if not args.no_serve:
         from pharos_amd import serve
         line["serving"] = serve.measure(duration_s=args.serve_seconds, rate_hz=args.serve_rate,
