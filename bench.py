@@ -462,7 +462,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (runs shard by scenario or epoch "
-              f"range with no exchange, and the adaptive gain's one scalar per 100-epoch window is all they would share)", file=sys.stderr)
+              f"range with no exchange; the adaptive gain's one scalar per 100-epoch window is all they would "
+              f"share)", file=sys.stderr)
         sys.exit(2)
 
     from pharos_amd.engine import DeviceBuffer, require_gpu
