@@ -114,6 +114,14 @@ class RunEvent(C.Structure):
 RUN_STATUS = {"ok": 0, "end": 1, "arg": 2, "nomem": 3, "internal": 4, "input": 5, "io": 6, "bundle": 7, "busy": 8}
 
 
+# pharos_trajectory_knot and pharos_cache_fault (csrc/pharos_host.h)
+KNOT_DTYPE = np.dtype([("t", "<f8"), ("pos", "<f8", 3), ("vel", "<f8", 3), ("acc", "<f8", 3)])
+
+
+class CacheFault(C.Structure):
+    _fields_ = [("code", C.c_uint32), ("reserved", C.c_uint32), ("offset", C.c_uint64), ("message", C.c_char * 256)]
+
+
 # pharos_synth_row as a numpy record (64 bytes, include/pharos_synth.h)
 ROW_DTYPE = np.dtype([("carrier_phase0", "<f8"), ("carrier_step", "<f8"), ("entry_phase0", "<f8"),
                       ("entries_per_sample", "<f8"), ("lead_entries", "<f8"), ("period0", "<u8"), ("table", "<u4"),
@@ -199,6 +207,9 @@ def lib():
         "pharos_scenario_nav_bit": (C.c_int, [P(ChannelSpec), I64, U32]),
         "pharos_run_abi_version": (U32, []),
         "pharos_run_source_digest": (U64, []),
+        "pharos_trajectory_read": (C.c_int, [VP, C.c_size_t, VP, C.c_size_t, P(C.c_size_t), C.c_char_p, C.c_size_t,
+                                             P(CacheFault)]),
+        "pharos_trajectory_write": (C.c_int, [C.c_char_p, VP, C.c_size_t, VP, C.c_size_t, P(C.c_size_t)]),
         "pharos_run_last_error": (C.c_char_p, []),
         "pharos_run_create": (C.c_int, [VP, C.c_size_t, P(RunOptions), P(VP)]),
         "pharos_run_info_get": (C.c_int, [VP, P(RunInfo)]),

@@ -19,7 +19,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("PHAROS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["synth_engine.hip"]
-CXX_SOURCES = ["signals.cpp", "scenario.cpp", "output.cpp", "run_api.cpp"]
+CXX_SOURCES = ["signals.cpp", "scenario.cpp", "output.cpp", "run_api.cpp", "trajectory.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off", "-fno-fast-math",

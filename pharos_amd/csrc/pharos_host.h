@@ -37,6 +37,31 @@ int pharos_secondary_code(uint32_t signal, uint32_t prn, uint8_t *out, uint32_t 
 int pharos_table_build(uint32_t signal, uint32_t prn, pharos_synth_table *out);
 void pharos_table_release(pharos_synth_table *t);
 
+// ----------------------------------------------------------------------------- the reference's cache files
+// A trajectory knot (domain/trajectory/trajectory.zig:6-11): time in s from the start, then ECEF position (m),
+// velocity (m/s) and acceleration (m/s^2).
+typedef struct pharos_trajectory_knot {
+    double t;
+    double pos[3], vel[3], acc[3];
+} pharos_trajectory_knot;
+
+// Why a cache file was refused: the reference's diagnostic code (diagnostics.zig:36-40: 600 header, 601 sources,
+// 602 truncated, 603 invalid value, 604 trailing bytes), the byte offset it names, and its message.
+typedef struct pharos_cache_fault {
+    uint32_t code;
+    uint32_t reserved;
+    uint64_t offset;
+    char message[256];
+} pharos_cache_fault;
+
+// A .pht trajectory file (src/pht/read.zig:8-15): 0 with its knots and name; 2 if an argument is null or `cap`
+// is below the knot count (then *count still says how many there are); 5 if the file is refused, with *fault.
+int pharos_trajectory_read(const uint8_t *bytes, size_t len, pharos_trajectory_knot *knots, size_t cap,
+                           size_t *count, char *name, size_t name_cap, pharos_cache_fault *fault);
+// The .pht bytes of a trajectory (src/pht/write.zig:5-9): 0, or 2 if `out` is null or shorter than *len.
+int pharos_trajectory_write(const char *name, const pharos_trajectory_knot *knots, size_t count, uint8_t *out,
+                            size_t cap, size_t *len);
+
 // ----------------------------------------------------------------------------- synthetic scenario stage
 // One channel of a synthetic scenario: the receiver is at a known range-rate model from the satellite, and the
 // epoch-rate stage turns that model into endpoints (carrier phase in cycles, transmit time) at every epoch edge,

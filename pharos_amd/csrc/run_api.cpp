@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/pharos_run.h"
+#include "cache_codec.h"
 #include "pharos_host.h"
 
 namespace {
@@ -337,16 +338,14 @@ int copy_bytes(const char *bytes, size_t n, uint8_t *buf, size_t cap, size_t *le
 
 // The bundle: a synthetic scenario (PHSC) -- or a .phb, recognised and refused.
 int parse_bundle(const uint8_t *b, size_t n, pharos_run *r) {
-    if (n >= 16 && memcmp(b, "PHB1", 4) == 0) {
-        uint32_t version;
-        uint64_t digest;
-        memcpy(&version, b + 4, 4);
-        memcpy(&digest, b + 8, 8);
-        if (version != 2) {
+    if (n >= cache::HEADER_LEN && memcmp(b, "PHB1", 4) == 0) {  // header.zig:16-30
+        uint64_t digest = 0;
+        const cache::HeaderFault fault = cache::check_header(b, n, "PHB1", SOURCE_DIGEST, &digest);
+        if (fault == cache::HEADER_VERSION) {
             g_thread_error.set("the bundle header version is not this build's; re-run pharos-pack on the scenario");
             return PHAROS_RUN_E_INPUT;
         }
-        if (digest != SOURCE_DIGEST) {
+        if (fault == cache::HEADER_SOURCES) {
             g_thread_error.set("the bundle was written by a Pharos built from other sources (%016llx); this build's "
                                "sources are %016llx; re-run pharos-pack on the scenario",
                                (unsigned long long)digest, (unsigned long long)SOURCE_DIGEST);
