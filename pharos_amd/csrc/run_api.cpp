@@ -395,15 +395,21 @@ int parse_bundle(const uint8_t *b, size_t n, pharos_run *r) {
     return PHAROS_RUN_OK;
 }
 
-}  // namespace
+// Exceptions stop at the C boundary: memory running out is E_NOMEM, anything else E_INTERNAL, each with a message.
+template <class F>
+int guarded(ErrorBuffer *err, F body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        if (err) err->set("out of host memory");
+        return PHAROS_RUN_E_NOMEM;
+    } catch (...) {
+        if (err) err->set("internal error");
+        return PHAROS_RUN_E_INTERNAL;
+    }
+}
 
-extern "C" {
-
-uint32_t pharos_run_abi_version(void) { return ABI_VERSION; }
-uint64_t pharos_run_source_digest(void) { return SOURCE_DIGEST; }
-const char *pharos_run_last_error(void) { return g_thread_error.text; }
-
-int pharos_run_create(const uint8_t *bundle, size_t len, const pharos_run_options *o, pharos_run **out) {
+int create_impl(const uint8_t *bundle, size_t len, const pharos_run_options *o, pharos_run **out) {
     g_thread_error.clear();
     if (!out) return PHAROS_RUN_E_ARG;
     *out = nullptr;
@@ -461,26 +467,7 @@ int pharos_run_create(const uint8_t *bundle, size_t len, const pharos_run_option
     return PHAROS_RUN_OK;
 }
 
-int pharos_run_info_get(const pharos_run *r, pharos_run_info *o) {
-    if (!r || !o) return PHAROS_RUN_E_ARG;
-    memset(o, 0, sizeof *o);
-    o->format_bits = 2u << r->format;
-    o->sample_rate_hz = (uint64_t)r->spec.samples_per_epoch * 1000;
-    o->center_hz = r->center_hz;
-    o->samples_per_epoch = r->spec.samples_per_epoch;
-    o->workers = r->options.workers ? r->options.workers : 1;
-    o->bytes_per_epoch = r->bpe;
-    o->total_epochs = r->total_epochs;
-    o->start_gps_ms = r->spec.start_gps_ms;
-    o->start_gps_sub_ms = r->spec.start_gps_sub_ms;
-    o->leap_change_gps_ms = 0;
-    o->leap_seconds = r->spec.leap_seconds;
-    o->leap_seconds_after = r->spec.leap_seconds;
-    return PHAROS_RUN_OK;
-}
-
-// Run.pull (engine/run.zig:225-256) behind pharos_run_pull (c_api/run.zig:142-165).
-int pharos_run_pull(pharos_run *r, uint8_t *buf, size_t cap, uint32_t max_epochs, pharos_run_block *o) {
+int pull_impl(pharos_run *r, uint8_t *buf, size_t cap, uint32_t max_epochs, pharos_run_block *o) {
     if (!r) return PHAROS_RUN_E_ARG;
     r->err.clear();
     if (!o || !buf) return PHAROS_RUN_E_ARG;
@@ -528,9 +515,7 @@ int pharos_run_pull(pharos_run *r, uint8_t *buf, size_t cap, uint32_t max_epochs
     return status;
 }
 
-// Run.seek (engine/run.zig:258-275) behind pharos_run_seek (c_api/run.zig:167-186).  The synthesis is a pure
-// function of an epoch's rows, so stepping to the target applies the commands due on the way and synthesises nothing.
-int pharos_run_seek(pharos_run *r, uint64_t target) {
+int seek_impl(pharos_run *r, uint64_t target) {
     if (!r) return PHAROS_RUN_E_ARG;
     r->err.clear();
     if (r->adaptive) {
@@ -563,9 +548,7 @@ int pharos_run_seek(pharos_run *r, uint64_t target) {
     return PHAROS_RUN_OK;
 }
 
-// pharos_run_submit (c_api/run.zig:188-200): decode (interface/c/command.zig:19-47), check
-// (simulation/model/command.zig:26-45) and queue (engine/commands.zig:20-28).
-int pharos_run_submit(pharos_run *r, const pharos_run_command *cp) {
+int submit_impl(pharos_run *r, const pharos_run_command *cp) {
     if (!r) return PHAROS_RUN_E_ARG;
     r->err.clear();
     if (!cp) return PHAROS_RUN_E_ARG;
@@ -608,17 +591,7 @@ int pharos_run_submit(pharos_run *r, const pharos_run_command *cp) {
     return PHAROS_RUN_OK;
 }
 
-int pharos_run_next_event(pharos_run *r, pharos_run_event *o) {
-    if (!r || !o || r->len == 0) return 0;
-    *o = r->ring[r->head];
-    r->head = (r->head + 1) % EVENT_CAPACITY;
-    r->len--;
-    return 1;
-}
-
-uint32_t pharos_run_events_dropped(const pharos_run *r) { return r ? r->dropped : 0; }
-
-int pharos_run_manifest_json(pharos_run *r, const pharos_run_manifest_fields *f, uint8_t *buf, size_t cap, size_t *len) {
+int manifest_impl(pharos_run *r, const pharos_run_manifest_fields *f, uint8_t *buf, size_t cap, size_t *len) {
     if (!r) return PHAROS_RUN_E_ARG;
     r->err.clear();
     const pharos_run_manifest_fields fields = f ? *f : pharos_run_manifest_fields{nullptr, nullptr, nullptr};
@@ -651,8 +624,7 @@ int pharos_run_manifest_json(pharos_run *r, const pharos_run_manifest_fields *f,
     return copy_bytes(o.data(), o.size(), buf, cap, len);
 }
 
-// pharos_run_tag_text (c_api/run.zig:234-243): the first sample is the run's entry epoch.
-int pharos_run_tag_text(pharos_run *r, uint8_t *buf, size_t cap, size_t *len) {
+int tag_impl(pharos_run *r, uint8_t *buf, size_t cap, size_t *len) {
     if (!r) return PHAROS_RUN_E_ARG;
     r->err.clear();
     char text[256];
@@ -667,12 +639,7 @@ int pharos_run_tag_text(pharos_run *r, uint8_t *buf, size_t cap, size_t *len) {
     return copy_bytes(text, n, buf, cap, len);
 }
 
-// pharos_run_to_file (c_api/run.zig:245-259, output/sink.zig:18-47): the rest of the run written from the current
-// epoch in blocks on the 100-epoch grid (after an off-grid seek the first block stops at the grid), each block
-// written while the next one is pulled into the other half of a double buffer; events raised while writing are
-// dropped (run.zig:292-294).  A failed write stops the pulls and is reported before a generation failure, with the
-// bytes on file in order.
-int pharos_run_to_file(pharos_run *r, const char *path, uint64_t *bytes_written) {
+int to_file_impl(pharos_run *r, const char *path, uint64_t *bytes_written) {
     if (!r) return PHAROS_RUN_E_ARG;
     r->err.clear();
     if (bytes_written) *bytes_written = 0;
@@ -731,6 +698,82 @@ int pharos_run_to_file(pharos_run *r, const char *path, uint64_t *bytes_written)
         return PHAROS_RUN_E_IO;
     }
     return failure;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t pharos_run_abi_version(void) { return ABI_VERSION; }
+uint64_t pharos_run_source_digest(void) { return SOURCE_DIGEST; }
+const char *pharos_run_last_error(void) { return g_thread_error.text; }
+
+int pharos_run_create(const uint8_t *bundle, size_t len, const pharos_run_options *o, pharos_run **out) {
+    return guarded(&g_thread_error, [&] { return create_impl(bundle, len, o, out); });
+}
+
+int pharos_run_info_get(const pharos_run *r, pharos_run_info *o) {
+    if (!r || !o) return PHAROS_RUN_E_ARG;
+    memset(o, 0, sizeof *o);
+    o->format_bits = 2u << r->format;
+    o->sample_rate_hz = (uint64_t)r->spec.samples_per_epoch * 1000;
+    o->center_hz = r->center_hz;
+    o->samples_per_epoch = r->spec.samples_per_epoch;
+    o->workers = r->options.workers ? r->options.workers : 1;
+    o->bytes_per_epoch = r->bpe;
+    o->total_epochs = r->total_epochs;
+    o->start_gps_ms = r->spec.start_gps_ms;
+    o->start_gps_sub_ms = r->spec.start_gps_sub_ms;
+    o->leap_change_gps_ms = 0;
+    o->leap_seconds = r->spec.leap_seconds;
+    o->leap_seconds_after = r->spec.leap_seconds;
+    return PHAROS_RUN_OK;
+}
+
+// Run.pull (engine/run.zig:225-256) behind pharos_run_pull (c_api/run.zig:142-165).
+int pharos_run_pull(pharos_run *r, uint8_t *buf, size_t cap, uint32_t max_epochs, pharos_run_block *o) {
+    return guarded((r ? &r->err : nullptr), [&] { return pull_impl(r, buf, cap, max_epochs, o); });
+}
+
+// Run.seek (engine/run.zig:258-275) behind pharos_run_seek (c_api/run.zig:167-186).  The synthesis is a pure
+// function of an epoch's rows, so stepping to the target applies the commands due on the way and synthesises nothing.
+int pharos_run_seek(pharos_run *r, uint64_t target) {
+    return guarded((r ? &r->err : nullptr), [&] { return seek_impl(r, target); });
+}
+
+// pharos_run_submit (c_api/run.zig:188-200): decode (interface/c/command.zig:19-47), check
+// (simulation/model/command.zig:26-45) and queue (engine/commands.zig:20-28).
+int pharos_run_submit(pharos_run *r, const pharos_run_command *cp) {
+    return guarded((r ? &r->err : nullptr), [&] { return submit_impl(r, cp); });
+}
+
+int pharos_run_next_event(pharos_run *r, pharos_run_event *o) {
+    if (!r || !o || r->len == 0) return 0;
+    *o = r->ring[r->head];
+    r->head = (r->head + 1) % EVENT_CAPACITY;
+    r->len--;
+    return 1;
+}
+
+uint32_t pharos_run_events_dropped(const pharos_run *r) { return r ? r->dropped : 0; }
+
+int pharos_run_manifest_json(pharos_run *r, const pharos_run_manifest_fields *f, uint8_t *buf, size_t cap,
+                             size_t *len) {
+    return guarded((r ? &r->err : nullptr), [&] { return manifest_impl(r, f, buf, cap, len); });
+}
+
+// pharos_run_tag_text (c_api/run.zig:234-243): the first sample is the run's entry epoch.
+int pharos_run_tag_text(pharos_run *r, uint8_t *buf, size_t cap, size_t *len) {
+    return guarded((r ? &r->err : nullptr), [&] { return tag_impl(r, buf, cap, len); });
+}
+
+// pharos_run_to_file (c_api/run.zig:245-259, output/sink.zig:18-47): the rest of the run written from the current
+// epoch in blocks on the 100-epoch grid (after an off-grid seek the first block stops at the grid), each block
+// written while the next one is pulled into the other half of a double buffer; events raised while writing are
+// dropped (run.zig:292-294).  A failed write stops the pulls and is reported before a generation failure, with the
+// bytes on file in order.
+int pharos_run_to_file(pharos_run *r, const char *path, uint64_t *bytes_written) {
+    return guarded((r ? &r->err : nullptr), [&] { return to_file_impl(r, path, bytes_written); });
 }
 
 const char *pharos_run_error(const pharos_run *r) { return r ? r->err.text : ""; }

@@ -627,7 +627,7 @@ static void build_turn(std::vector<double> &t) {
     }
 }
 
-extern "C" int pharos_synth_turn_table(double *out, uint32_t count) {
+static int turn_table_body(double *out, uint32_t count) {
     g_error[0] = 0;
     if (!out || count > 4096) { set_error("turn table: null output or more than 4096 entries"); return ST_ARG; }
     std::vector<double> t;
@@ -684,7 +684,7 @@ static int static_scratch(int device, uint8_t **out) {
     return ST_OK;
 }
 
-extern "C" int pharos_synth_create(const pharos_synth_config *config, const pharos_synth_table *tables, uint32_t table_count,
+static int create_body(const pharos_synth_config *config, const pharos_synth_table *tables, uint32_t table_count,
                                    int device, pharos_synth_engine **out) {
     g_error[0] = 0;
     if (!out || !config || (table_count && !tables)) { set_error("null argument"); return ST_ARG; }
@@ -896,7 +896,7 @@ static int launch_window_tiles(pharos_synth_engine *e, uint64_t first_epoch, uin
     return ST_OK;
 }
 
-extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+static int window_body(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
                                    const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
                                    uint8_t *out, uint64_t *clipped, void *stream_) {
     g_error[0] = 0;
@@ -939,7 +939,7 @@ extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch,
 
 // The plain form with no allocation: the same kernels, with the per-tile clip counts and per-epoch totals in the
 // device's static scratch array instead of memory the engine allocates.  Calls on one device share that array.
-extern "C" int pharos_synth_window_static(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs,
+static int window_static_body(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs,
                                           uint32_t channels, const pharos_synth_row *rows, const uint8_t *masks,
                                           uint64_t mask_len, float gain, uint8_t *out, uint64_t *clipped,
                                           void *stream_) {
@@ -1116,7 +1116,7 @@ static uint32_t cu_count(int device) {
     return (uint32_t)n;
 }
 
-extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning,
+static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning,
                                          void *stream_, pharos_synth_batch_handle **out) {
     g_error[0] = 0;
     if (!out || (njobs && !jobs)) { set_error("null argument"); return ST_ARG; }
@@ -1394,7 +1394,7 @@ extern "C" int pharos_synth_batch_abort(pharos_synth_batch_handle *h) {
     return ST_OK;
 }
 
-extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
+static int batch_wait_body(pharos_synth_batch_handle *h) {
     if (!h) { set_error("null handle"); return ST_ARG; }
     (void)hipSetDevice(h->device);
     const uint32_t njobs = (uint32_t)h->jobs.size();
@@ -1500,7 +1500,7 @@ static Written write_at(int fd, const uint8_t *buf, size_t n, uint64_t offset) {
     return {0, done};
 }
 
-extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path, uint64_t *bytes_written) {
+static int to_file_body(const pharos_synth_job *J, const char *path, uint64_t *bytes_written) {
     g_error[0] = 0;
     if (bytes_written) *bytes_written = 0;
     if (!J || !path) { set_error("null argument"); return ST_ARG; }
@@ -1760,7 +1760,7 @@ __global__ __launch_bounds__(PREC_THREADS) void precision_kernel(const DevEngine
     }
 }
 
-extern "C" int pharos_synth_precision(pharos_synth_engine *e, const pharos_synth_row *rows, uint64_t nrows,
+static int precision_body(pharos_synth_engine *e, const pharos_synth_row *rows, uint64_t nrows,
                                       pharos_synth_precision_report *report, double *code_error_per_table,
                                       void *stream_) {
     g_error[0] = 0;
@@ -1802,7 +1802,7 @@ extern "C" int pharos_synth_precision(pharos_synth_engine *e, const pharos_synth
     return ST_OK;
 }
 
-extern "C" int pharos_synth_batch(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning, void *stream) {
+static int batch_body(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning, void *stream) {
     pharos_synth_batch_handle *h = nullptr;
     int st = pharos_synth_batch_submit(jobs, njobs, tuning, stream, &h);
     if (st) return st;
@@ -1856,7 +1856,7 @@ static void launch_window(pharos_synth_engine *e, const dim3 &grid, hipStream_t 
     }
 }
 
-extern "C" int pharos_synth_graph_create(const pharos_synth_job *job, pharos_synth_graph **out) {
+static int graph_create_body(const pharos_synth_job *job, pharos_synth_graph **out) {
     g_error[0] = 0;
     if (!job || !out) { set_error("null argument"); return ST_ARG; }
     *out = nullptr;
@@ -1955,7 +1955,7 @@ extern "C" int pharos_synth_graph_launch(pharos_synth_graph *g, void *stream_) {
 }
 
 // Waits for the replay on `stream` and hands back the job's AGC state, gain changes and clip total.
-extern "C" int pharos_synth_graph_finish(pharos_synth_graph *g, void *stream_) {
+static int graph_finish_body(pharos_synth_graph *g, void *stream_) {
     g_error[0] = 0;
     if (!g) { set_error("null graph"); return ST_ARG; }
     hipStream_t stream = (hipStream_t)stream_;
@@ -2098,4 +2098,73 @@ extern "C" int pharos_synth_device_sync(int device) {
     HIP_TRY(hipSetDevice(device), ST_ARG);
     HIP_TRY(hipDeviceSynchronize(), ST_INTERNAL);
     return ST_OK;
+}
+
+// ------------------------------------------------------------------ the C boundary
+// Exceptions stop at the C entry points: host memory running out is status 3, anything else status 4, each with a
+// message.
+template <class F>
+int host_guarded(F body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        set_error("out of host memory");
+        return ST_NOMEM;
+    } catch (...) {
+        set_error("internal error");
+        return ST_INTERNAL;
+    }
+}
+
+extern "C" int pharos_synth_create(const pharos_synth_config *config, const pharos_synth_table *tables, uint32_t table_count,
+                                   int device, pharos_synth_engine **out) {
+    return host_guarded([&] { return create_body(config, tables, table_count, device, out); });
+}
+
+extern "C" int pharos_synth_window(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+                                   const pharos_synth_row *rows, const uint8_t *masks, uint64_t mask_len, float gain,
+                                   uint8_t *out, uint64_t *clipped, void *stream_) {
+    return host_guarded([&] { return window_body(e, first_epoch, epochs, channels, rows, masks, mask_len, gain, out, clipped, stream_); });
+}
+
+extern "C" int pharos_synth_window_static(pharos_synth_engine *e, uint64_t first_epoch, uint32_t epochs,
+                                          uint32_t channels, const pharos_synth_row *rows, const uint8_t *masks,
+                                          uint64_t mask_len, float gain, uint8_t *out, uint64_t *clipped,
+                                          void *stream_) {
+    return host_guarded([&] { return window_static_body(e, first_epoch, epochs, channels, rows, masks, mask_len, gain, out, clipped, stream_); });
+}
+
+extern "C" int pharos_synth_batch_submit(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning,
+                                         void *stream_, pharos_synth_batch_handle **out) {
+    return host_guarded([&] { return batch_submit_body(jobs, njobs, tuning, stream_, out); });
+}
+
+extern "C" int pharos_synth_batch_wait(pharos_synth_batch_handle *h) {
+    return host_guarded([&] { return batch_wait_body(h); });
+}
+
+extern "C" int pharos_synth_to_file(const pharos_synth_job *J, const char *path, uint64_t *bytes_written) {
+    return host_guarded([&] { return to_file_body(J, path, bytes_written); });
+}
+
+extern "C" int pharos_synth_precision(pharos_synth_engine *e, const pharos_synth_row *rows, uint64_t nrows,
+                                      pharos_synth_precision_report *report, double *code_error_per_table,
+                                      void *stream_) {
+    return host_guarded([&] { return precision_body(e, rows, nrows, report, code_error_per_table, stream_); });
+}
+
+extern "C" int pharos_synth_batch(const pharos_synth_job *jobs, uint32_t njobs, const pharos_synth_tuning *tuning, void *stream) {
+    return host_guarded([&] { return batch_body(jobs, njobs, tuning, stream); });
+}
+
+extern "C" int pharos_synth_graph_create(const pharos_synth_job *job, pharos_synth_graph **out) {
+    return host_guarded([&] { return graph_create_body(job, out); });
+}
+
+extern "C" int pharos_synth_graph_finish(pharos_synth_graph *g, void *stream_) {
+    return host_guarded([&] { return graph_finish_body(g, stream_); });
+}
+
+extern "C" int pharos_synth_turn_table(double *out, uint32_t count) {
+    return host_guarded([&] { return turn_table_body(out, count); });
 }
