@@ -503,6 +503,14 @@ __device__ __forceinline__ uint32_t bit_insert(uint32_t mask, uint32_t a, uint32
 
 constexpr uint32_t ANY_WAVE = 0xffffffffu;  // one copy for both waves, the slot address in a register
 
+// s_waitcnt operand (gfx9 encoding) that waits for LDS and scalar-memory counts to reach 0 and leaves the vector
+// memory (vmcnt 63) and export (expcnt 7) counts unconstrained.
+constexpr uint32_t LGKM_WAIT_ALL = 0xC07Fu;
+#ifndef PHAROS_LDS_WAIT_PER_BATCH
+#define PHAROS_LDS_WAIT_PER_BATCH 1
+#endif
+constexpr bool LDS_WAIT_PER_BATCH = PHAROS_LDS_WAIT_PER_BATCH;
+
 template <uint32_t AXIS, bool LEAD, uint32_t WAVE>
 __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[STEPS], uint64_t pos, uint64_t step,
                                                 float cre, float cim, float wre, float wim, uint64_t lead_add,
@@ -560,6 +568,9 @@ __device__ __forceinline__ void channel_samples(float (&ar)[STEPS], float (&ai)[
 #pragma unroll
     for (uint32_t n = 0; n < NB; n++) {
         const uint32_t cur = n & 1, nxt = cur ^ 1;
+        // One wait for every LDS read in flight (this batch's values and the next batch's entries, both issued a
+        // batch ago) in place of a wait before each value read: the waits are instructions the wave issues.
+        if (LDS_WAIT_PER_BATCH) __builtin_amdgcn_s_waitcnt(LGKM_WAIT_ALL);
         if (n + 1 < NB) values(va[nxt], vl[nxt], vx[nxt], vy[nxt]);
         if (n + 2 < NB) addresses(va[cur], vl[cur]);
         // Stage 3: accumulate (kernel.zig:241-258) and advance the rotor (Rotor.advance, rotor.zig:52-58).
