@@ -30,6 +30,15 @@ __device__ __forceinline__ uint4 gload_u4(const void *p) {
 }
 
 // ------------------------------------------------------------------ f64 turn-table sine/cosine (rotor.zig:76-86)
+// x / d correctly rounded for a constant d, in one multiply and two FMAs instead of the division sequence:
+// q0 = x * RN(1/d) is within one ulp of x / d, and by Markstein's theorem (IBM J. Res. Dev. 34(1), 1990) the
+// residual step q0 + (x - d q0) RN(1/d) rounds to the correctly rounded quotient.  Holds for normal x with no
+// overflow; the series below divide r^2 in [0, 2.4e-6].
+__device__ __forceinline__ double div_by_const(double x, double d, double inv_d) {
+    const double q0 = x * inv_d;
+    return fma(fma(-q0, d, x), inv_d, q0);
+}
+
 __device__ __forceinline__ void sincos64(double phi, const double *__restrict__ turn, double &c, double &s) {
     double t = phi - floor(phi);
     double u = t * 4096.0;
@@ -37,8 +46,8 @@ __device__ __forceinline__ void sincos64(double phi, const double *__restrict__ 
     uint32_t i = (uint32_t)fu & 4095u;
     double r = (u - fu) * (2.0 * 3.14159265358979323846 / 4096.0);
     double r2 = r * r;
-    double cr = 1.0 - r2 * (0.5 - r2 * (1.0 / 24.0 - r2 / 720.0));
-    double sr = r * (1.0 - r2 * (1.0 / 6.0 - r2 * (1.0 / 120.0 - r2 / 5040.0)));
+    double cr = 1.0 - r2 * (0.5 - r2 * (1.0 / 24.0 - div_by_const(r2, 720.0, 1.0 / 720.0)));
+    double sr = r * (1.0 - r2 * (1.0 / 6.0 - r2 * (1.0 / 120.0 - div_by_const(r2, 5040.0, 1.0 / 5040.0))));
     const f64x2_t e = *gmem(reinterpret_cast<const f64x2_t *>(turn + 2 * i));
     const double ec = e.x, es = e.y;
     c = ec * cr - es * sr;
