@@ -150,6 +150,26 @@ int pharos_synth_derive(const pharos_synth_segment *segments, uint32_t block_epo
                         uint32_t epochs, uint32_t channels, int64_t run_start_ms, double run_start_sub_ms,
                         uint32_t samples_per_epoch, pharos_synth_row *rows, void *stream);
 
+/* A channel's amplitude from an epoch on: one step of the run's power levels (engine/run.zig:335-365 applies the
+ * commands due at each window's first epoch, engine/commands.zig:38-62, and the scenario's power schedule,
+ * simulation/state/power.zig:33-38, and the rows' amplitudes follow from the levels, parameters.zig:78-81).  The host
+ * turns each signal-power or noise-floor command and each schedule entry into the amplitude of every channel whose
+ * C/N0 it changes; the device applies them at their epochs, so a launch over many windows keeps each command's
+ * exact epoch with no host round trip.  amp NaN: back to the segment's own amplitude. */
+typedef struct pharos_synth_level {
+    uint64_t epoch;              /* first epoch of the run with this amplitude */
+    uint32_t channel;            /* channel index of the run */
+    float amp;                   /* row amplitude from `epoch` on, or NaN for the segment's */
+} pharos_synth_level;            /* 16 bytes */
+
+/* pharos_synth_derive with the run's power levels: an active row of epoch e and channel c takes the amplitude of the
+ * last level with channel c and epoch <= e, if any (and not NaN).  levels (device, level_count entries) are sorted
+ * by channel, then epoch; a NULL levels with level_count 0 is pharos_synth_derive. */
+int pharos_synth_derive_levels(const pharos_synth_segment *segments, uint32_t block_epochs, uint64_t first_epoch,
+                               uint32_t epochs, uint32_t channels, int64_t run_start_ms, double run_start_sub_ms,
+                               uint32_t samples_per_epoch, const pharos_synth_level *levels, uint32_t level_count,
+                               pharos_synth_row *rows, void *stream);
+
 /* Plain form: one launch synthesises `epochs` epochs from first_epoch under one gain, like one window of
  * Run.runJob(.produce) (run.zig:367-380).  rows (device) is [epochs][channels]; masks (device) holds mask_len
  * bytes; out (device) receives epochs * bytes_per_epoch bytes.  clipped (host, optional) receives one count per

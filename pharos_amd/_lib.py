@@ -134,6 +134,9 @@ SEGMENT_DTYPE = np.dtype([("phase", "<f8", 4), ("delay", "<f8", 4), ("amp", "<f8
                           ("mask_count", "<u4"), ("code_ms", "<u4"), ("entries_per_ms", "<u4"), ("table", "<u4"),
                           ("lead_bits", "u1"), ("reserved", "u1", 3)])
 assert SEGMENT_DTYPE.itemsize == 136
+# pharos_synth_level: a channel's amplitude from an epoch on (the run's power levels)
+LEVEL_DTYPE = np.dtype([("epoch", "<u8"), ("channel", "<u4"), ("amp", "<f4")])
+assert LEVEL_DTYPE.itemsize == 16
 INACTIVE = 0xFFFFFFFF
 
 SIGNALS = ["L1CA", "L1C", "L2C", "L5", "B1C", "B1I", "B2I", "B3I", "B2a", "B2b", "E1", "E5a", "E5b", "E6", "G1", "G2"]
@@ -225,6 +228,8 @@ def lib():
         "pharos_run_destroy": (None, [VP]),
         "pharos_scenario_segments": (I64, [P(ScenarioSpec), P(ChannelSpec), U64, U32, U32, VP, VP, U64]),
         "pharos_synth_derive": (C.c_int, [VP, U32, U64, U32, U32, I64, C.c_double, U32, VP, VP]),
+        "pharos_synth_derive_levels": (C.c_int, [VP, U32, U64, U32, U32, I64, C.c_double, U32, VP, U32, VP, VP]),
+        "pharos_scenario_amplitude": (C.c_double, [C.c_double, U32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

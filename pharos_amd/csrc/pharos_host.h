@@ -110,6 +110,9 @@ void pharos_scenario_endpoint(const pharos_scenario_spec *scenario, const pharos
 int64_t pharos_scenario_segments(const pharos_scenario_spec *scenario, const pharos_channel_spec *channels,
                                  uint64_t first_epoch, uint32_t epochs, uint32_t block_epochs,
                                  pharos_synth_segment *segs, uint8_t *masks, uint64_t mask_cap);
+// A row amplitude from a C/N0 in dB-Hz at samples_per_epoch samples per ms (parameters.zig:78-81), the value the
+// stand-in's rows and segments carry.
+double pharos_scenario_amplitude(double cn0_dbhz, uint32_t samples_per_epoch);
 // The stand-in's nav bit `bit` of frame `frame` of a channel (the message content SymbolSource reads; 0 or 1).
 int pharos_scenario_nav_bit(const pharos_channel_spec *ch, int64_t frame, uint32_t bit);
 // parameters.derive with the channel's constants; returns 1 for an active row.

@@ -103,12 +103,11 @@ struct pharos_run {
     std::vector<pharos_synth_table> tables;
     pharos_synth_engine *engine = nullptr;
     void *stream = nullptr;
-    void *d_segs = nullptr, *d_masks = nullptr, *d_rows = nullptr, *d_out = nullptr;
+    void *d_segs = nullptr, *d_masks = nullptr, *d_rows = nullptr, *d_out = nullptr, *d_levels = nullptr;
     uint64_t masks_cap = 0, bpe = 0;
-    // The host stage's output for one block: its segments and masks, and the channel levels they were made with.
+    // The host stage's output for one block, at the scenario's own levels: its segments and masks.
     struct Prepared {
         uint64_t bstart = UINT64_MAX;
-        std::vector<double> cn0;
         std::vector<pharos_synth_segment> segs;
         std::vector<uint8_t> masks;
         int64_t used = -1;
@@ -125,6 +124,12 @@ struct pharos_run {
     double noise_floor = DEFAULT_NOISE_FLOOR;
     std::vector<double> cn0_default, cn0_commanded;  // commanded: NaN when the channel is at its default
     std::vector<pharos_run_command> queue;           // pending commands, in submission order
+    // The run's power levels as amplitude steps for the device derive (pharos_synth_level), sorted by channel then
+    // epoch, with each channel's current amplitude; uploaded to d_levels when they change.
+    std::vector<pharos_synth_level> levels;
+    std::vector<float> amp_now;
+    uint32_t d_levels_cap = 0;
+    bool levels_dirty = true;
     uint64_t taken = 0;
     pharos_run_applied applied[COMMAND_CAPACITY];
     size_t applied_len = 0;
@@ -151,7 +156,7 @@ void release(pharos_run *r) {
     if (!r) return;
     if (r->next.valid()) r->next.wait();
     if (r->stream) pharos_synth_stream_sync(r->stream);
-    for (void *p : {r->d_segs, r->d_masks, r->d_rows, r->d_out})
+    for (void *p : {r->d_segs, r->d_masks, r->d_rows, r->d_out, r->d_levels})
         if (p) pharos_synth_free(p);
     if (r->stream) pharos_synth_stream_destroy(r->stream);
     if (r->engine) pharos_synth_destroy(r->engine);
@@ -170,7 +175,10 @@ void restart(pharos_run *r) {
     for (size_t c = 0; c < r->channels.size(); c++) {
         r->cn0_default[c] = r->channels[c].cn0_dbhz;
         r->cn0_commanded[c] = NAN;
+        r->amp_now[c] = (float)pharos_scenario_amplitude(r->channels[c].cn0_dbhz, r->spec.samples_per_epoch);
     }
+    r->levels.clear();
+    r->levels_dirty = true;
 }
 
 // Power.setNoiseFloor (simulation/state/power.zig:62-72): levels follow the floor's change.
@@ -224,6 +232,19 @@ void apply_due(pharos_run *r, uint64_t epoch) {
         }
     }
     r->queue.swap(kept);
+    // Every channel whose amplitude the commands changed takes a level step at this epoch (kept sorted by channel).
+    for (size_t c = 0; c < r->channels.size(); c++) {
+        const float amp = (float)pharos_scenario_amplitude(r->cn0(c), r->spec.samples_per_epoch);
+        if (amp == r->amp_now[c]) continue;
+        r->amp_now[c] = amp;
+        pharos_synth_level step{epoch, (uint32_t)c, amp};
+        auto at = std::upper_bound(r->levels.begin(), r->levels.end(), step,
+                                   [](const pharos_synth_level &a, const pharos_synth_level &b) {
+                                       return a.channel != b.channel ? a.channel < b.channel : a.epoch < b.epoch;
+                                   });
+        r->levels.insert(at, step);
+        r->levels_dirty = true;
+    }
 }
 
 // Run.windowAt (engine/run.zig:325-331): to the next AGC evaluation, block edge or pending command.
@@ -234,46 +255,55 @@ uint32_t window_at(const pharos_run *r, uint64_t first, uint64_t limit) {
     return (uint32_t)n;
 }
 
-// The host stage for the 100-epoch block starting at bstart with the given channel levels (any thread).
-pharos_run::Prepared prepare(const pharos_run *r, uint64_t bstart, std::vector<double> cn0) {
+// The host stage for the 100-epoch block starting at bstart, at the scenario's own levels (any thread): the power
+// levels reach the rows on the device (pharos_synth_derive_levels), so a command never re-prepares a block.
+pharos_run::Prepared prepare(const pharos_run *r, uint64_t bstart) {
     pharos_run::Prepared p;
     const uint32_t nch = (uint32_t)r->channels.size();
-    std::vector<pharos_channel_spec> now = r->channels;
-    for (uint32_t c = 0; c < nch; c++) now[c].cn0_dbhz = cn0[c];
     const uint32_t blen = (uint32_t)std::min<uint64_t>(WINDOW, r->total_epochs - bstart);
     p.bstart = bstart;
-    p.cn0 = std::move(cn0);
     p.segs.resize(std::max<uint32_t>(nch, 1));
     p.masks.resize(r->masks_cap);
-    p.used = pharos_scenario_segments(&r->spec, now.data(), bstart, blen, blen, p.segs.data(), p.masks.data(),
-                                      r->masks_cap);
+    p.used = pharos_scenario_segments(&r->spec, r->channels.data(), bstart, blen, blen, p.segs.data(),
+                                      p.masks.data(), r->masks_cap);
     return p;
 }
 
-std::vector<double> levels(const pharos_run *r) {
-    std::vector<double> v(r->channels.size());
-    for (size_t c = 0; c < v.size(); c++) v[c] = r->cn0(c);
-    return v;
+// The level steps to the device when they changed since the last upload.
+int upload_levels(pharos_run *r) {
+    if (!r->levels_dirty) return 0;
+    const uint32_t n = (uint32_t)r->levels.size();
+    if (n > r->d_levels_cap) {
+        const uint32_t cap = std::max<uint32_t>(n, 2 * r->d_levels_cap);
+        void *p = nullptr;
+        if (int st = pharos_synth_malloc(0, (uint64_t)cap * sizeof(pharos_synth_level), &p)) return st;
+        if (r->d_levels) pharos_synth_free(r->d_levels);
+        r->d_levels = p;
+        r->d_levels_cap = cap;
+    }
+    if (n)
+        if (int st = pharos_synth_memcpy(r->d_levels, r->levels.data(), (uint64_t)n * sizeof(pharos_synth_level), 0))
+            return st;
+    r->levels_dirty = false;
+    return 0;
 }
 
 // The window's samples on the GPU: the segments of the 100-epoch block holding the window (fitted over the whole
-// block, so a row does not depend on where windows split it, at a command or a seek) with the channels' current
-// C/N0, the rows derived on the device from the block's start, the plain window, and the bytes copied into `out`.
-// While the GPU runs the window, a worker prepares the next block's segments and masks at the same levels (the
-// double-buffered rows of engine/run.zig:382-407); a command that changes a level before that block is used makes
-// it prepared again.  Returns a status and the clip total.
+// block, so a row does not depend on where windows split it, at a command or a seek), the rows derived on the device
+// from the block's start with the run's power levels applied at their epochs, the plain window, and the bytes copied
+// into `out`.  While the GPU runs the window, a worker prepares the next block's segments and masks (the
+// double-buffered rows of engine/run.zig:382-407).  Returns a status and the clip total.
 int synthesise(pharos_run *r, uint64_t first, uint32_t count, uint8_t *out, uint64_t *clipped) {
     const uint32_t nch = (uint32_t)r->channels.size();
     const uint64_t bstart = first - first % WINDOW;
     const uint32_t blen = (uint32_t)std::min<uint64_t>(WINDOW, r->total_epochs - bstart);
     const uint32_t skip = (uint32_t)(first - bstart);
-    std::vector<double> now = levels(r);
-    if (r->current.bstart != bstart || r->current.cn0 != now) {
+    if (r->current.bstart != bstart) {
         if (r->next.valid()) {
             pharos_run::Prepared p = r->next.get();
-            r->current = p.bstart == bstart && p.cn0 == now ? std::move(p) : prepare(r, bstart, now);
+            r->current = p.bstart == bstart ? std::move(p) : prepare(r, bstart);
         } else {
-            r->current = prepare(r, bstart, now);
+            r->current = prepare(r, bstart);
         }
     }
     const int64_t used = r->current.used;
@@ -285,14 +315,16 @@ int synthesise(pharos_run *r, uint64_t first, uint32_t count, uint8_t *out, uint
     const uint64_t following = bstart + WINDOW;
     if (!r->next.valid() && following < r->total_epochs) {
         try {
-            r->next = std::async(std::launch::async, prepare, r, following, now);
+            r->next = std::async(std::launch::async, prepare, r, following);
         } catch (...) {  // no thread to be had: the next block is prepared when it is reached
         }
     }
+    if (!st) st = upload_levels(r);
     if (!st && nch)
-        st = pharos_synth_derive((const pharos_synth_segment *)r->d_segs, blen, bstart, skip + count, nch,
-                                 r->spec.start_gps_ms, r->spec.start_gps_sub_ms, r->spec.samples_per_epoch,
-                                 (pharos_synth_row *)r->d_rows, r->stream);
+        st = pharos_synth_derive_levels((const pharos_synth_segment *)r->d_segs, blen, bstart, skip + count, nch,
+                                        r->spec.start_gps_ms, r->spec.start_gps_sub_ms, r->spec.samples_per_epoch,
+                                        (const pharos_synth_level *)r->d_levels, (uint32_t)r->levels.size(),
+                                        (pharos_synth_row *)r->d_rows, r->stream);
     const float gain = r->adaptive ? (float)r->agc.gain : r->constant_gain;
     if (!st)
         st = pharos_synth_window(r->engine, first, count, nch, (const pharos_synth_row *)r->d_rows + (size_t)skip * nch,
@@ -451,6 +483,7 @@ int create_impl(const uint8_t *bundle, size_t len, const pharos_run_options *o, 
     r->clips.resize(WINDOW);
     r->cn0_default.resize(nch);
     r->cn0_commanded.resize(nch);
+    r->amp_now.resize(nch);
     if (pharos_synth_stream_create(0, &r->stream) ||
         pharos_synth_malloc(0, sizeof(pharos_synth_segment) * std::max<uint32_t>(nch, 1), &r->d_segs) ||
         pharos_synth_malloc(0, r->masks_cap, &r->d_masks) ||

@@ -182,9 +182,7 @@ void cubic_through(const double *tau, const double *y, double *coef) {
     for (int k = 0; k < 4; k++) coef[k] = c[k];
 }
 
-double amplitude(const pharos_channel_spec *c, uint32_t spe) {
-    return (double)(float)sqrt(2.0 * pow(10.0, c->cn0_dbhz / 10.0) / (1000.0 * (double)spe));  // parameters.zig:78-81
-}
+double amplitude(const pharos_channel_spec *c, uint32_t spe) { return pharos_scenario_amplitude(c->cn0_dbhz, spe); }
 
 // parameters.derive (parameters.zig:51-70) of one epoch from its two endpoints.
 int derive_from(const pharos_scenario_spec *s, const pharos_channel_spec *c, const pharos_signal_constants &k,
@@ -362,6 +360,10 @@ int64_t pharos_scenario_window(const pharos_scenario_spec *s, const pharos_chann
         used += len;
     }
     return (int64_t)used;
+}
+
+double pharos_scenario_amplitude(double cn0_dbhz, uint32_t samples_per_epoch) {
+    return (double)(float)sqrt(2.0 * pow(10.0, cn0_dbhz / 10.0) / (1000.0 * (double)samples_per_epoch));  // parameters.zig:78-81
 }
 
 }  // extern "C"

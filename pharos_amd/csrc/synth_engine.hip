@@ -124,7 +124,7 @@ constexpr uint32_t SEG_BLOCK = 256;
 // The segment holding `item`: the cursor's segment or the next one, else from the table of every SEG_BLOCK-th
 // item's segment and a walk forward (segments hold their items in order, so a few steps at most; a binary search
 // over the rest if the walk runs long).
-__device__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
+__device__ __forceinline__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
     const auto segs = gmem(S.segs);
     if (hint < S.nsegs && segs[hint].first_item <= item &&
         (hint + 1 == S.nsegs || segs[hint + 1].first_item > item))
@@ -184,7 +184,7 @@ __device__ __forceinline__ void publish_gain(const BatchState &S, uint32_t seg, 
 // segment's gain.  The job's AGC state is ordinary memory written by one segment's last tile after another: this
 // tile read the gain word the previous writer published after a release fence, so an acquire fence here makes that
 // writer's state visible (once per segment, not per tile).
-__device__ void agc_account(const BatchState &S, uint64_t first_epoch, uint32_t spe, uint32_t job, const DevSegment &seg,
+__device__ __forceinline__ void agc_account(const BatchState &S, uint64_t first_epoch, uint32_t spe, uint32_t job, const DevSegment &seg,
                             uint32_t s, unsigned long long clipped) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     DevAgc &G = S.agc[job];
@@ -244,9 +244,27 @@ __host__ __device__ inline int64_t mod_floor64(int64_t a, int64_t b) {
 
 // One row (epoch e, channel c): the endpoints at the epoch's two edges from the segment's polynomials, then
 // parameters.derive (parameters.zig:51-70) with ifPhase (:72-76) in the same operations and order.
+// The power levels (pharos_synth_level, sorted by channel then epoch): the amplitude of the last level of channel c
+// at or before epoch `ep`, else `amp`.
+__device__ __forceinline__ float level_amp(const pharos_synth_level *levels, uint32_t count, uint32_t c, uint64_t ep,
+                                           float amp) {
+    uint32_t lo = 0, hi = count;  // the first level after (c, ep) is levels[lo] once lo == hi
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        const uint32_t lc = gmem(levels)[mid].channel;
+        if (lc < c || (lc == c && gmem(levels)[mid].epoch <= ep)) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo == 0 || gmem(levels)[lo - 1].channel != c) return amp;
+    const float a = gmem(levels)[lo - 1].amp;
+    return __builtin_isnan(a) ? amp : a;
+}
+
 __global__ __launch_bounds__(256) void derive_kernel(const pharos_synth_segment *segs, uint32_t block_epochs,
                                                      uint64_t first_epoch, uint32_t epochs, uint32_t channels,
-                                                     int64_t start_ms, double sub, uint32_t spe, pharos_synth_row *rows) {
+                                                     int64_t start_ms, double sub, uint32_t spe,
+                                                     const pharos_synth_level *levels, uint32_t level_count,
+                                                     pharos_synth_row *rows) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)epochs * channels) return;
     const uint32_t e = (uint32_t)(i / channels), c = (uint32_t)(i % channels);
@@ -283,6 +301,7 @@ __global__ __launch_bounds__(256) void derive_kernel(const pharos_synth_segment 
             r.period0 = k;
             r.table = S.table;
             r.amp = (float)(S.amp[1] * tau + S.amp[0]);
+            if (level_count) r.amp = level_amp(levels, level_count, c, first_epoch + e, r.amp);
             r.lead_bits = S.lead_bits;
             r.lead_entries = S.lead_bits ? S.lead_entries : 0.0;
             // The row's masks from the segment's range; a period outside it leaves the row no mask bytes, which the
@@ -299,11 +318,13 @@ __global__ __launch_bounds__(256) void derive_kernel(const pharos_synth_segment 
     rows[i] = r;
 }
 
-extern "C" int pharos_synth_derive(const pharos_synth_segment *segs, uint32_t block_epochs, uint64_t first_epoch,
-                                   uint32_t epochs, uint32_t channels, int64_t run_start_ms, double run_start_sub_ms,
-                                   uint32_t samples_per_epoch, pharos_synth_row *rows, void *stream) {
+extern "C" int pharos_synth_derive_levels(const pharos_synth_segment *segs, uint32_t block_epochs,
+                                          uint64_t first_epoch, uint32_t epochs, uint32_t channels,
+                                          int64_t run_start_ms, double run_start_sub_ms, uint32_t samples_per_epoch,
+                                          const pharos_synth_level *levels, uint32_t level_count,
+                                          pharos_synth_row *rows, void *stream) {
     g_error[0] = 0;
-    if (!rows || (channels && !segs)) { set_error("null argument"); return ST_ARG; }
+    if (!rows || (channels && !segs) || (level_count && !levels)) { set_error("null argument"); return ST_ARG; }
     if (block_epochs == 0 || samples_per_epoch == 0) { set_error("block_epochs and samples_per_epoch must be positive"); return ST_ARG; }
     if (!(run_start_sub_ms >= 0.0 && run_start_sub_ms < 1.0)) { set_error("run_start_sub_ms must be in [0, 1)"); return ST_ARG; }
     const uint64_t n = (uint64_t)epochs * channels;
@@ -311,9 +332,17 @@ extern "C" int pharos_synth_derive(const pharos_synth_segment *segs, uint32_t bl
     const uint64_t blocks = (n + 255) / 256;
     if (blocks > 0x7fffffffu) { set_error("too many rows in one call"); return ST_ARG; }
     hipLaunchKernelGGL(derive_kernel, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream, segs, block_epochs,
-                       first_epoch, epochs, channels, run_start_ms, run_start_sub_ms, samples_per_epoch, rows);
+                       first_epoch, epochs, channels, run_start_ms, run_start_sub_ms, samples_per_epoch, levels,
+                       level_count, rows);
     HIP_TRY(hipGetLastError(), ST_INTERNAL);
     return ST_OK;
+}
+
+extern "C" int pharos_synth_derive(const pharos_synth_segment *segs, uint32_t block_epochs, uint64_t first_epoch,
+                                   uint32_t epochs, uint32_t channels, int64_t run_start_ms, double run_start_sub_ms,
+                                   uint32_t samples_per_epoch, pharos_synth_row *rows, void *stream) {
+    return pharos_synth_derive_levels(segs, block_epochs, first_epoch, epochs, channels, run_start_ms,
+                                      run_start_sub_ms, samples_per_epoch, nullptr, 0, rows, stream);
 }
 
 // The batched form's pre-pass: RowPrep of every active row of a job, one thread per row.
@@ -335,7 +364,7 @@ __global__ __launch_bounds__(256) void row_prep_kernel(const pharos_synth_row *r
 // The checks sit off the common path: a wait that ends within 15 polls runs only the gain loads, the give-up word
 // is read every 16th poll, and the host's word (a read across the host link) every 256th once a wait has passed
 // 1024 polls.
-__device__ uint32_t wait_gain(const BatchState &S, uint32_t s) {
+__device__ __forceinline__ uint32_t wait_gain(const BatchState &S, uint32_t s) {
     unsigned long long w = __hip_atomic_load(&S.seg_gain[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (uint32_t polls = 0; !(w & READY_BIT); polls++) {
         if ((polls & 15u) == 15u) {

@@ -777,7 +777,7 @@ struct TileArgs {
 // GainFn by every thread before quantising (the persistent kernel waits there for the AGC of the previous window).
 // Returns the clipped components of the tile.
 template <uint32_t F, class GainFn, class BeforeLoop, class AfterSums>
-__device__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, BeforeLoop before_loop,
+__device__ __forceinline__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, BeforeLoop before_loop,
                                AfterSums after_sums, uint32_t *err) {
     DevEngine E = load_engine(A.E);
     if (A.chunk_cap && A.chunk_cap < E.chunk) E.chunk = A.chunk_cap;

@@ -234,10 +234,11 @@ def run_plain(scenario, first_epoch=0, epochs=None, device=0, format_override=No
     return result, clips, agc.gain, agc.adjustments()
 
 
-def derive_rows(scenario, first_epoch, epochs, block_epochs=100, device=0, stream=None, segments=None):
+def derive_rows(scenario, first_epoch, epochs, block_epochs=100, device=0, stream=None, segments=None, levels=None):
     """Rows on the device from the scenario's per-block segments (pharos_synth_derive): the host computes one
-    segment per channel and block plus the masks; the device evaluates every epoch's row.  Returns (rows, masks,
-    mask_len) as device buffers."""
+    segment per channel and block plus the masks; the device evaluates every epoch's row.  levels (LEVEL_DTYPE,
+    sorted by channel then epoch) are power-level steps the device applies to the rows' amplitudes at their epochs
+    (pharos_synth_derive_levels).  Returns (rows, masks, mask_len) as device buffers."""
     if segments is None:
         segments = scenario.segments(first_epoch, epochs, block_epochs)
     segs, masks = segments
@@ -245,9 +246,12 @@ def derive_rows(scenario, first_epoch, epochs, block_epochs=100, device=0, strea
     d_segs = DeviceBuffer.from_array(segs, device)
     d_masks = DeviceBuffer.from_array(masks, device)
     d_rows = DeviceBuffer(max(1, epochs * nch) * 64, device)
-    check(_lib.lib().pharos_synth_derive(d_segs.ptr if nch else None, block_epochs, first_epoch, epochs, nch,
-                                         scenario.start_gps_ms, scenario.start_gps_sub_ms, scenario.samples_per_epoch,
-                                         d_rows.ptr, stream.handle if stream else None))
+    nlev = 0 if levels is None else len(levels)
+    d_levels = DeviceBuffer.from_array(np.ascontiguousarray(levels, dtype=_lib.LEVEL_DTYPE), device) if nlev else None
+    check(_lib.lib().pharos_synth_derive_levels(d_segs.ptr if nch else None, block_epochs, first_epoch, epochs, nch,
+                                                scenario.start_gps_ms, scenario.start_gps_sub_ms,
+                                                scenario.samples_per_epoch, d_levels.ptr if nlev else None, nlev,
+                                                d_rows.ptr, stream.handle if stream else None))
     if stream:
         stream.sync()
     else:
@@ -258,16 +262,17 @@ def derive_rows(scenario, first_epoch, epochs, block_epochs=100, device=0, strea
 class Job:
     """One run in a batch: its engine, device rows/masks/output and its gain state.  Rows come from host rows and
     masks (rows=, masks=, or the scenario's host stage), or with derive=True from the scenario's segments, evaluated
-    on the device."""
+    on the device with the power-level steps `levels`, if any."""
 
     def __init__(self, scenario, first_epoch=0, epochs=None, device=0, rows=None, masks=None, adjustments_cap=4096,
-                 derive=False, block_epochs=100):
+                 derive=False, block_epochs=100, levels=None):
         self.scenario = scenario
         self.first_epoch = first_epoch
         self.epochs = scenario.epochs if epochs is None else epochs
         self.engine = Engine(scenario, device)
         if derive:
-            self.rows, self.masks, self.mask_len = derive_rows(scenario, first_epoch, self.epochs, block_epochs, device)
+            self.rows, self.masks, self.mask_len = derive_rows(scenario, first_epoch, self.epochs, block_epochs, device,
+                                                               levels=levels)
         else:
             if rows is None:
                 rows, masks = scenario.window(first_epoch, self.epochs)

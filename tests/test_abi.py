@@ -80,6 +80,7 @@ int main(void) {
   S(pharos_synth_job) F(pharos_synth_job, engine) F(pharos_synth_job, rows) F(pharos_synth_job, mask_len)
       F(pharos_synth_job, gain_mode) F(pharos_synth_job, constant_gain) F(pharos_synth_job, agc)
       F(pharos_synth_job, adjustments_cap) F(pharos_synth_job, clipped) E
+  S(pharos_synth_level) F(pharos_synth_level, epoch) F(pharos_synth_level, channel) F(pharos_synth_level, amp) E
   printf("\"pharos_synth_tuning\": [%zu]\n}\n", sizeof(pharos_synth_tuning));
   return 0;
 }
@@ -98,10 +99,11 @@ def test_record_layouts_match_the_c_compiler(tmp_path):
     py_name = {"samples_per_epoch": "samples_per_epoch"}
     for cname, entry in layout.items():
         size, fields = entry[0], entry[1:]
-        if cname == "pharos_synth_row":
-            assert _lib.ROW_DTYPE.itemsize == size
+        if cname in ("pharos_synth_row", "pharos_synth_level"):
+            dt = _lib.ROW_DTYPE if cname == "pharos_synth_row" else _lib.LEVEL_DTYPE
+            assert dt.itemsize == size, cname
             for f, off in fields:
-                assert _lib.ROW_DTYPE.fields[f][1] == off, f
+                assert dt.fields[f][1] == off, (cname, f)
             continue
         T = ctypes_of[cname]
         assert C.sizeof(T) == size, cname

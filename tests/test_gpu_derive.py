@@ -52,3 +52,50 @@ def test_run_from_device_rows_equals_the_oracle():
         want, _, _, _ = oracle_window(sc, e, 1, gain=1.0, rows=rows[e:e + 1], masks=masks)
         got = job.out.to_numpy(sc.bytes_per_epoch, e * sc.bytes_per_epoch)
         assert np.array_equal(got, want), e
+
+
+def level_steps(nch, spe):
+    """Power-level steps as the run API emits them: amplitudes of C/N0 values at mid-block and block-edge epochs,
+    a return to the segment's amplitude (NaN), a step before the derived range and a channel with none."""
+    amp = lambda cn0: np.float32(_lib.lib().pharos_scenario_amplitude(cn0, spe))
+    steps = [(40, 0, amp(38.0)), (137, 0, amp(52.5)), (250, 0, np.float32(np.nan)), (251, 0, amp(41.0)),
+             (150, 3, amp(30.0)), (999, 3, amp(60.0)), (100, nch - 1, amp(47.25)), (1234, nch - 1, amp(35.0))]
+    lv = np.array(steps, dtype=_lib.LEVEL_DTYPE)
+    return lv[np.lexsort((lv["epoch"], lv["channel"]))]
+
+
+def apply_levels(rows, levels, first):
+    """The rows' amplitudes under the steps: the last step of the channel at or before each row's epoch."""
+    out = rows.copy()
+    for e in range(rows.shape[0]):
+        for c in range(rows.shape[1]):
+            if out[e, c]["table"] == _lib.INACTIVE:
+                continue
+            mine = levels[(levels["channel"] == c) & (levels["epoch"] <= first + e)]
+            if len(mine) and not np.isnan(mine[-1]["amp"]):
+                out[e, c]["amp"] = mine[-1]["amp"]
+    return out
+
+
+def test_device_levels_apply_at_their_epochs():
+    sc = S.c1(seed=9, epochs=1500)
+    first, epochs, block = 100, 1300, 100
+    nch = len(sc.channels)
+    levels = level_steps(nch, sc.samples_per_epoch)
+    print(f"derive levels: c1, {len(levels)} steps, {epochs} epochs from {first}", flush=True)
+    segs, masks = sc.segments(first, epochs, block)
+    d_rows, _, _ = derive_rows(sc, first, epochs, block, segments=(segs, masks), levels=levels)
+    got = d_rows.to_numpy(epochs * nch * 64).view(_lib.ROW_DTYPE).reshape(epochs, -1)
+    plain = O.derive_segments(segs, sc.start_gps_ms, 0.0, first, epochs, block, sc.samples_per_epoch)
+    want = apply_levels(plain, levels, first)
+    assert got.tobytes() == want.tobytes()
+    changed = want["amp"] != plain["amp"]
+    assert changed[:, 0].any() and changed[:, 3].any() and changed[:, nch - 1].any() and not changed[:, 1].any()
+    # A run derived with levels synthesises as the oracle does from the same rows (under a constant gain).
+    sc.gain_db = 0.0
+    job = Job(sc, first_epoch=first, epochs=epochs, derive=True, block_epochs=block, levels=levels)
+    (job,) = run_batch([job])
+    for e in (37, 151, 1133):
+        want_b, _, _, _ = oracle_window(sc, first + e, 1, gain=1.0, rows=want[e:e + 1], masks=masks)
+        got_b = job.out.to_numpy(sc.bytes_per_epoch, e * sc.bytes_per_epoch)
+        assert np.array_equal(got_b, want_b), e
