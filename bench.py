@@ -33,6 +33,8 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
+from pharos_amd import tune  # noqa: E402
+
 METRIC = "channel-samples/sec + roofline %, batched vs plain on 1 GPU; p50/p99 run latency"
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 (vector, packed)
 PEAK_HBM_GBS = 8000.0
@@ -165,7 +167,7 @@ def _reset(job):
 
 def timed_batch(jobs, shape, stream):
     """(wall seconds, kernel seconds by HIP events on the launch stream) of one batched launch over `jobs` with the
-    launch shape (workgroups per CU, channels per pass)."""
+    launch shape (workgroups per CU, channels per pass, pharos_synth_tuning flags)."""
     import ctypes as C
     from pharos_amd import _lib
     from pharos_amd.engine import Event
@@ -173,7 +175,7 @@ def timed_batch(jobs, shape, stream):
     for j in jobs:
         _reset(j)
     arr = (_lib.SynthJob * len(jobs))(*[j.record() for j in jobs])
-    tuning = _lib.SynthTuning(shape[0], shape[1], 0, 0)
+    tuning = _lib.SynthTuning(shape[0], shape[1], 0, shape[2])
     e0, e1 = Event(), Event()
     h = C.c_void_p()
     L.pharos_synth_device_sync(0)
@@ -224,13 +226,14 @@ def timed_plain(jobs, stream):
 def launch_shape(args, scenarios, stream):
     """The batched launch shape: given on the command line, or chosen by the autotuner on a short sample of the
     same workload (pharos_amd/tune.py)."""
-    if args.workgroups_per_cu or args.channels_per_pass or args.no_tune:
-        return (args.workgroups_per_cu, args.channels_per_pass), {"source": "command line"}
-    from pharos_amd import tune
+    if args.workgroups_per_cu or args.channels_per_pass or args.no_tune or args.wide_tiles:
+        flags = tune.WIDE_TILES if args.wide_tiles else 0
+        return (args.workgroups_per_cu, args.channels_per_pass, flags), {"source": "command line"}
     t = tune.tune(scenarios, sample_epochs=args.tune_epochs, stream=stream)
-    return tuple(t["best"]), {"source": "autotuner", "key": t["key"], "best": {"workgroups_per_cu": t["best"][0],
-                              "channels_per_pass": t["best"][1]}, "sample_epochs": t["sample_epochs"],
-                              "candidates_ms": t["times_ms"], "seconds": t["seconds"]}
+    w, c, f = t["best"]
+    return (w, c, f), {"source": "autotuner", "key": t["key"], "best": {"workgroups_per_cu": w, "channels_per_pass": c,
+                       "tile_samples": 8192 if f & tune.WIDE_TILES else 4096}, "sample_epochs": t["sample_epochs"],
+                       "candidates_ms": t["times_ms"], "seconds": t["seconds"]}
 
 
 WORKLOADS = {
@@ -456,6 +459,7 @@ def main():
     ap.add_argument("--workgroups-per-cu", type=int, default=0, help="launch shape (skips the autotuner)")
     ap.add_argument("--channels-per-pass", type=int, default=0, help="launch shape (skips the autotuner)")
     ap.add_argument("--no-tune", action="store_true", help="use the library's default launch shape")
+    ap.add_argument("--wide-tiles", action="store_true", help="launch shape with 8192-sample tiles (skips the autotuner)")
     ap.add_argument("--tune-epochs", type=int, default=300, help="epochs of the autotuner's sample")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device three times (a known-size reference for HBM counters)")

@@ -11,13 +11,24 @@ namespace pharos {
 constexpr uint32_t RESEED = 512;     // rotor.zig:5
 constexpr uint32_t LANES16 = 16;     // rotor.zig:10 (chains x lanes)
 constexpr uint32_t STEPS = RESEED / LANES16;  // 32 advances per block
+// Two tile sizes.  A narrow tile is 8 rotor blocks (4096 samples, 128 threads, four workgroups per CU): every form
+// runs it.  A wide tile is 16 blocks (8192 samples, 256 threads, two workgroups per CU), for the batched form of
+// engines with several tiles per epoch: a tile's setup, barriers and gain wait then serve twice the samples.
 constexpr uint32_t TILE_BLOCKS = 8;
+constexpr uint32_t WIDE_BLOCKS = 16;
 constexpr uint32_t THREADS = TILE_BLOCKS * LANES16;  // 128 = two wave64
 constexpr uint32_t TILE = TILE_BLOCKS * RESEED;      // 4096 samples
+template <uint32_t TB> struct TileGeo {
+    static constexpr uint32_t BLOCKS = TB;
+    static constexpr uint32_t THREADS = TB * LANES16;
+    static constexpr uint32_t TILE = TB * RESEED;
+    static constexpr uint32_t WAVES = TB * LANES16 / 64;
+    // LDS per workgroup that keeps four narrow, or two wide, workgroups on a CU's 160 KiB
+    static constexpr uint32_t LDS_LIMIT = TB == TILE_BLOCKS ? 40960u : 81920u;
+};
 constexpr uint32_t FORMAT_ANY = 0xff;  // a batch kernel instance whose jobs may have any output format
 constexpr uint32_t MAX_CHUNK = 32;   // channels staged per LDS pass (at most THREADS / 4: one uint4 of a row per thread)
-constexpr uint32_t GROUP = THREADS / MAX_CHUNK;  // threads that set up and stage one channel of a pass
-constexpr uint32_t LDS_LIMIT = 40960;  // LDS per workgroup that keeps four workgroups on a CU's 160 KiB
+constexpr uint32_t LDS_LIMIT = TileGeo<TILE_BLOCKS>::LDS_LIMIT;
 constexpr uint32_t LEAD_ALLOW = 64;  // entries a leading component may be ahead
 constexpr double RATE_MARGIN = 1.031; // derive() admits 3 % (parameters.zig:46,55)
 constexpr uint32_t VT_BYTES = 128;   // value table: 16 x (re, im) f32, entries from the table's count on zero
@@ -35,7 +46,8 @@ struct DevTable {
     uint32_t per, ext_len;
     uint32_t count, axis;
     uint32_t tdm_periods, tdm_half;
-    uint32_t pilot_len, w_max; // w_max: most entries a tile may stage from this table
+    uint32_t pilot_len, w_max; // w_max: most entries a narrow tile may stage from this table
+    uint32_t w_max_wide, pad;  // the same for a wide tile (0: the table cannot stage a wide tile)
     double vre[8], vim[8];
 };
 
@@ -51,7 +63,7 @@ struct DevEngine {
     uint32_t chunk;            // channels staged per pass
     uint32_t lds_bytes;        // dynamic LDS the kernel needs for this engine
     uint32_t out_bytes;        // LDS bytes of a tile's output units
-    uint32_t pad;
+    uint32_t wide;             // 1: this view's fields are for wide tiles
 };
 
 // Per row (epoch, channel), computed once for all of the row's tiles by the batched form's pre-pass: the lane powers
@@ -62,24 +74,24 @@ struct RowPrep {
     uint32_t pad[2];
 };  // 272 B
 
-// Per channel, per tile (LDS).
-struct ChanRec {
+// Per channel, per tile of TB rotor blocks (LDS).
+template <uint32_t TB> struct ChanRecT {
     double pw_c[LANES16], pw_s[LANES16];  // w1^l, the rotor's f64 lane powers (rotor.zig:25-30)
     uint64_t pl[LANES16];      // l * eps_q: lane l's code-position offset from its block's first sample
-    double c0_c[TILE_BLOCKS], c0_s[TILE_BLOCKS];  // sincos64 of each block's start phase (rotor.zig:37-38)
+    double c0_c[TB], c0_s[TB]; // sincos64 of each block's start phase (rotor.zig:37-38)
     // Block b's first-sample code position in the per-sample loop's form: the low word is the Q32.32 fraction; the
     // high word is the LDS address of the staged byte of that sample's entry (the channel's stage base + the
     // entry's offset in the staged window), so a lane's position is pb[b] + pl[l].
-    uint64_t pb[TILE_BLOCKS];
+    uint64_t pb[TB];
     uint64_t step;             // 16 * eps_q: code advance between a lane's samples
     float w_re, w_im;          // f32 sincos64(16 step) (rotor.zig:31-33)
     uint32_t active, shape, axis;
     uint32_t lead_q, lead_whole, lead_bits;
     uint32_t pad[2];
 };
-static_assert(sizeof(ChanRec) % 16 == 0, "ChanRec keeps 16-byte alignment");
-static_assert(THREADS == MAX_CHUNK * GROUP && TILE_BLOCKS % GROUP == 0 && LANES16 % GROUP == 0,
-              "a pass sets up each channel with one group of GROUP threads");
+using ChanRec = ChanRecT<TILE_BLOCKS>;
+static_assert(sizeof(ChanRecT<TILE_BLOCKS>) % 16 == 0 && sizeof(ChanRecT<WIDE_BLOCKS>) % 16 == 0,
+              "a channel record keeps 16-byte alignment");
 
 // Error bits the kernels raise (read back by the host).
 enum : uint32_t {

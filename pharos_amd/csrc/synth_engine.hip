@@ -91,7 +91,8 @@ __global__ __launch_bounds__(THREADS, 2) void synth_window_kernel(const DevEngin
     A.chunk_cap = 0;
     A.prep = prep ? prep + (size_t)e * channels : nullptr;
     // The gain is the argument, or a device word another kernel of the same stream wrote (the graph form).
-    uint32_t clips = synth_tile<F>(A, smem, [&]() { return gain_dev ? *gmem(gain_dev) : gain; }, []() {}, []() {}, err);
+    uint32_t clips = synth_tile<F, TILE_BLOCKS>(A, smem, [&]() { return gain_dev ? *gmem(gain_dev) : gain; }, []() {},
+                                                []() {}, err);
     if (threadIdx.x == 0) clip_partial[(size_t)e * E->tiles + tile] = clips;
 }
 
@@ -386,8 +387,9 @@ __device__ __forceinline__ uint32_t wait_gain(const BatchState &S, uint32_t s) {
     return (uint32_t)w;
 }
 
-template <uint32_t F>
-__global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
+template <uint32_t F, uint32_t TB>
+__global__ __launch_bounds__(TileGeo<TB>::THREADS, TB == TILE_BLOCKS ? 2 : 1) void synth_batch_kernel(BatchState S) {
+    constexpr uint32_t THREADS = TileGeo<TB>::THREADS;
     extern __shared__ __align__(16) uint8_t smem[];
     // Queue words live after the largest layout of any job (synth_tile's reduction words sit below them).
     // q[0..1]: the item, q[2]: its segment, q[3]: the gain broadcast.
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(THREADS, 2) void synth_batch_kernel(BatchState S) {
                 }
             }
         };
-        const uint32_t clips = synth_tile<F>(A, smem, gain_fn, before_loop, after_sums, S.err);
+        const uint32_t clips = synth_tile<F, TB>(A, smem, gain_fn, before_loop, after_sums, S.err);
         if (threadIdx.x == 0) {
             // One atomic counts the tile and its clips.  Under the adaptive gain the last tile of the segment gets
             // the segment's total and runs the AGC step; under a constant gain nothing waits for the count.
@@ -517,6 +519,8 @@ struct pharos_synth_engine {
     pharos_synth_config cfg{};
     DevEngine view{};          // host copy with device pointers
     DevEngine *d_engine = nullptr;
+    DevEngine view_wide{};     // the same engine for wide tiles (tiles 0: the engine has none)
+    DevEngine *d_engine_wide = nullptr;
     std::vector<void *> owned; // device allocations
     uint64_t device_bytes = 0;
     uint32_t *d_err = nullptr;
@@ -740,6 +744,9 @@ static int create_body(const pharos_synth_config *config, const pharos_synth_tab
     e->cfg = c;
     const uint32_t spe = c.samples_per_epoch;
     const uint32_t t_eff = std::min<uint32_t>(TILE, (spe + RESEED - 1) / RESEED * RESEED);
+    // Wide tiles, for engines with more than one narrow tile per epoch whose tables allow the longer windows.
+    const uint32_t t_wide = std::min<uint32_t>(TileGeo<WIDE_BLOCKS>::TILE, (spe + RESEED - 1) / RESEED * RESEED);
+    bool wide_ok = spe > TILE;
 
     // Everything the engine reads from device memory goes in one allocation, uploaded with one copy: per table its
     // extended entries (plus 32 zero bytes) and TDM pilot, then the table descriptors, the engine record and the
@@ -748,7 +755,7 @@ static int create_body(const pharos_synth_config *config, const pharos_synth_tab
     std::vector<DevTable> dt(table_count);
     std::vector<uint64_t> ext_off(table_count), pilot_off(table_count, 0);
     uint64_t total = 0;
-    uint32_t w_slot_max = 0;
+    uint32_t w_slot_max = 0, w_slot_max_wide = 0;
     for (uint32_t k = 0; k < table_count; k++) {
         const pharos_synth_table &t = tables[k];
         if (!t.states || t.entries_per_period == 0 || (t.value_count != 4 && t.value_count != 8) || t.entries_per_ms == 0) {
@@ -776,10 +783,12 @@ static int create_body(const pharos_synth_config *config, const pharos_synth_tab
             engine_free(e);
             return ST_ARG;
         }
+        const uint64_t w_max_wide = (uint64_t)ceil((double)t_wide * eps_max) + LEAD_ALLOW + 8;
+        if (w_max_wide > 3ull * per) wide_ok = false;
         DevTable &d = dt[k];
         memset(&d, 0, sizeof d);
         d.per = per;
-        d.ext_len = per + (uint32_t)w_max;
+        d.ext_len = per + (uint32_t)(wide_ok ? std::max(w_max, w_max_wide) : w_max);
         d.count = t.value_count;
         bool any_re = false, any_im = false;
         for (int v = 0; v < 8; v++) {
@@ -790,6 +799,7 @@ static int create_body(const pharos_synth_config *config, const pharos_synth_tab
         }
         d.axis = !any_im ? AXIS_RE : (!any_re ? AXIS_IM : AXIS_BOTH);  // code_tables.zig:89-99
         d.w_max = (uint32_t)w_max;
+        d.w_max_wide = wide_ok ? (uint32_t)w_max_wide : 0u;
         ext_off[k] = total;
         total = align(total + d.ext_len + 32);
         if (t.tdm_periods) {
@@ -801,10 +811,13 @@ static int create_body(const pharos_synth_config *config, const pharos_synth_tab
         }
         // Staging copies whole 16-byte units from e0 & ~15: up to 15 bytes before the window and 16 after it.
         w_slot_max = std::max<uint32_t>(w_slot_max, (uint32_t)w_max + 32);
+        w_slot_max_wide = std::max<uint32_t>(w_slot_max_wide, (uint32_t)w_max_wide + 32);
     }
     const uint64_t tables_off = total;
     total = align(total + sizeof(DevTable) * std::max<uint32_t>(table_count, 1));
     const uint64_t engine_off = total;
+    total = align(total + sizeof(DevEngine));
+    const uint64_t engine_wide_off = total;
     total = align(total + sizeof(DevEngine));
     const uint64_t err_off = total;
     total += 64;
@@ -856,7 +869,25 @@ static int create_body(const pharos_synth_config *config, const pharos_synth_tab
         engine_free(e);
         return ST_ARG;
     }
+    e->view.wide = 0;
     memcpy(blob.data() + engine_off, &e->view, sizeof(DevEngine));
+    // The wide view: 8192-sample tiles, two workgroups per CU.
+    e->view_wide = e->view;
+    e->view_wide.wide = 1;
+    e->view_wide.tiles = 0;
+    if (wide_ok) {
+        using W = TileGeo<WIDE_BLOCKS>;
+        const uint32_t w_slot = (std::max<uint32_t>(w_slot_max_wide, 16) + 31) / 64 * 64 + 32;
+        const uint32_t wbudget = W::LDS_LIMIT - 64 - 16 - (uint32_t)sizeof(g_wave_vt_wide) - ROWS_BYTES;
+        const uint32_t wper_ch = (uint32_t)sizeof(ChanRecT<WIDE_BLOCKS>) + VT_BYTES + w_slot;
+        const uint32_t wchunk = std::max<uint32_t>(1, std::min<uint32_t>(wbudget / wper_ch, MAX_CHUNK));
+        e->view_wide.tiles = (spe + W::TILE - 1) / W::TILE;
+        e->view_wide.w_slot = w_slot;
+        e->view_wide.chunk = wchunk;
+        e->view_wide.lds_bytes = std::max<uint32_t>(ROWS_BYTES + wchunk * wper_ch, sum_bytes<WIDE_BLOCKS>()) + 64;
+        if (e->view_wide.lds_bytes > 160 * 1024) e->view_wide.tiles = 0;
+    }
+    memcpy(blob.data() + engine_wide_off, &e->view_wide, sizeof(DevEngine));
     if (hipMemcpy(base, blob.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
         set_error("engine upload failed");
         engine_free(e);
@@ -869,6 +900,7 @@ static int create_body(const pharos_synth_config *config, const pharos_synth_tab
         return ST_INTERNAL;
     }
     e->d_engine = (DevEngine *)(base + engine_off);
+    e->d_engine_wide = (DevEngine *)(base + engine_wide_off);
     e->d_err = (uint32_t *)(base + err_off);
     *out = e;
     return ST_OK;
@@ -1184,10 +1216,14 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     }
     // One launch: a single-format batch runs the kernel instance with only that format's quantiser, a mixed one
     // the instance that picks each job's quantiser per tile.
+    // Wide tiles when the launch shape asks for them and every job's engine has them.
+    bool wide = tuning && (tuning->flags & PHAROS_SYNTH_TUNE_WIDE_TILES);
+    for (uint32_t j = 0; j < njobs && wide; j++) wide = jobs[j].engine->view_wide.tiles != 0;
+    auto view_of = [&](const pharos_synth_engine *e) -> const DevEngine & { return wide ? e->view_wide : e->view; };
     uint32_t batch_format = jobs[0].engine->view.format, batch_lds = 0;
     for (uint32_t j = 0; j < njobs; j++) {
         if (jobs[j].engine->view.format != batch_format) batch_format = FORMAT_ANY;
-        batch_lds = std::max(batch_lds, jobs[j].engine->view.lds_bytes);
+        batch_lds = std::max(batch_lds, view_of(jobs[j].engine).lds_bytes);
     }
     // Queue order: window w of a job of W windows at (w + 1/2) / W of the launch, so every job's windows spread over
     // the whole launch and the jobs end together.  (Window by window across jobs, a job with many short windows ran
@@ -1216,7 +1252,7 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
         s.job = j;
         s.epoch_off = w.off;
         s.epochs = w.epochs;
-        s.tiles = jobs[j].engine->view.tiles;
+        s.tiles = view_of(jobs[j].engine).tiles;
         items += (uint64_t)s.epochs * s.tiles;
         const uint32_t idx = (uint32_t)segs.size();
         segs.push_back(s);
@@ -1242,7 +1278,7 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
         const pharos_synth_job &J = jobs[j];
         DevJob &d = dj[j];
         memset(&d, 0, sizeof d);
-        d.engine = J.engine->d_engine;
+        d.engine = wide ? J.engine->d_engine_wide : J.engine->d_engine;
         d.rows = J.rows;
         d.masks = J.masks;
         d.out = J.out;
@@ -1280,7 +1316,7 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     std::vector<size_t> prep_at(njobs, SIZE_MAX);
     size_t prep_rows = 0;
     for (uint32_t j = 0; j < njobs; j++) {
-        if (jobs[j].engine->view.tiles < 2) continue;
+        if (view_of(jobs[j].engine).tiles < 2) continue;
         prep_at[j] = prep_rows;
         prep_rows += (size_t)jobs[j].epochs * jobs[j].channels;
     }
@@ -1364,25 +1400,30 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     {
         const uint32_t lds = S.queue_words_offset + 16;
         const void *fn = nullptr;
+#define PHAROS_BATCH_KERNEL(FMT) \
+    (wide ? (const void *)synth_batch_kernel<FMT, WIDE_BLOCKS> : (const void *)synth_batch_kernel<FMT, TILE_BLOCKS>)
         switch (batch_format) {
-        case PHAROS_SYNTH_IQ2: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ2>; break;
-        case PHAROS_SYNTH_IQ4: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ4>; break;
-        case PHAROS_SYNTH_IQ8: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ8>; break;
-        case PHAROS_SYNTH_IQ16: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_IQ16>; break;
-        case PHAROS_SYNTH_F32: fn = (const void *)synth_batch_kernel<PHAROS_SYNTH_F32>; break;
-        default: fn = (const void *)synth_batch_kernel<FORMAT_ANY>; break;
+        case PHAROS_SYNTH_IQ2: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ2); break;
+        case PHAROS_SYNTH_IQ4: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ4); break;
+        case PHAROS_SYNTH_IQ8: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ8); break;
+        case PHAROS_SYNTH_IQ16: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ16); break;
+        case PHAROS_SYNTH_F32: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_F32); break;
+        default: fn = PHAROS_BATCH_KERNEL(FORMAT_ANY); break;
         }
+#undef PHAROS_BATCH_KERNEL
+        const uint32_t threads = wide ? TileGeo<WIDE_BLOCKS>::THREADS : THREADS;
         uint32_t per_cu = tuning && tuning->workgroups_per_cu ? tuning->workgroups_per_cu : 0;
         if (!per_cu) {
             int occ = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, THREADS, lds) != hipSuccess || occ <= 0) occ = 2;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, lds) != hipSuccess || occ <= 0) occ = wide ? 1 : 2;
             per_cu = (uint32_t)occ;
         }
+        if (wide) per_cu = std::min<uint32_t>(per_cu, 2);  // a wide workgroup takes half a CU's LDS at most
         uint64_t grid = (uint64_t)cu_count(device) * per_cu;
         if (grid > items) grid = items;
         if (grid == 0) grid = 1;
         void *args[] = {&S};
-        if (hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(THREADS), args, lds, h->stream) != hipSuccess) {
+        if (hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(threads), args, lds, h->stream) != hipSuccess) {
             set_error("batch launch failed");
             (void)hipStreamSynchronize(h->stream);
             batch_free(h);

@@ -199,9 +199,9 @@ __device__ __forceinline__ float2 lds_load_f2(uint32_t a) {
 // ------------------------------------------------------------------ LDS layout of a work item
 // The channel region (records, value tables, staged entries) and, after the channel loop, the tile's sums
 // (TILE f32 I then TILE f32 Q) share the same bytes.
-struct Lds {
+template <uint32_t TB> struct Lds {
     pharos_synth_row *rows;  // the chunk's rows (at most MAX_CHUNK)
-    ChanRec *rec;
+    ChanRecT<TB> *rec;
     uint8_t *vt;      // chunk x VT_BYTES: 16 float2 values per channel
     uint8_t *stage;   // chunk x w_slot
     float *sum_re, *sum_im;
@@ -209,15 +209,16 @@ struct Lds {
     uint32_t vt_base, stage_base;  // byte offsets from smem
 };
 
-constexpr uint32_t SUM_BYTES = TILE * 8;
+template <uint32_t TB> constexpr uint32_t sum_bytes() { return TileGeo<TB>::TILE * 8; }
+constexpr uint32_t SUM_BYTES = sum_bytes<TILE_BLOCKS>();
 constexpr uint32_t ROWS_BYTES = MAX_CHUNK * (uint32_t)sizeof(pharos_synth_row);
 
-__device__ __forceinline__ Lds lds_layout(uint8_t *smem, const DevEngine &E) {
-    Lds L;
+template <uint32_t TB> __device__ __forceinline__ Lds<TB> lds_layout(uint8_t *smem, const DevEngine &E) {
+    Lds<TB> L;
     L.rows = reinterpret_cast<pharos_synth_row *>(smem);
     uint32_t off = ROWS_BYTES;
-    L.rec = reinterpret_cast<ChanRec *>(smem + off);
-    off += E.chunk * (uint32_t)sizeof(ChanRec);
+    L.rec = reinterpret_cast<ChanRecT<TB> *>(smem + off);
+    off += E.chunk * (uint32_t)sizeof(ChanRecT<TB>);
     L.vt = smem + off;
     L.vt_base = off;
     off += E.chunk * VT_BYTES;
@@ -225,8 +226,8 @@ __device__ __forceinline__ Lds lds_layout(uint8_t *smem, const DevEngine &E) {
     L.stage_base = off;
     off += E.chunk * E.w_slot;
     L.sum_re = reinterpret_cast<float *>(smem);
-    L.sum_im = L.sum_re + TILE;
-    off = off > SUM_BYTES ? off : SUM_BYTES;
+    L.sum_im = L.sum_re + TileGeo<TB>::TILE;
+    off = off > sum_bytes<TB>() ? off : sum_bytes<TB>();
     L.red = reinterpret_cast<uint32_t *>(smem + off);
     return L;
 }
@@ -340,10 +341,10 @@ __device__ __forceinline__ void row_prep(const pharos_synth_row &row, const doub
 //   - lane powers and value-table entries j * PER .. j * PER + PER - 1, PER = 16 / G: the powers of Rotor.init
 //     (rotor.zig:21-35) each by the reference's f64 recursion, the values f32(f64(amp) * value) (kernel.zig:104-110),
 //   - dwords j, j + G, ... of the staged window,
-// and thread 0 the record's scalars.  A pass of up to THREADS / 8 channels gives each channel 8 threads, a larger
-// one 4.
-template <uint32_t G>
-__device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *stage, uint32_t stage_addr,
+// and thread 0 the record's scalars.  A pass of up to THREADS / (2 G) channels gives each channel 2 G threads, a
+// larger one G = THREADS / MAX_CHUNK; TB is the tile's rotor blocks.
+template <uint32_t G, uint32_t TB>
+__device__ __forceinline__ void chan_group(ChanRecT<TB> &R, float2 *vt, uint32_t *stage, uint32_t stage_addr,
                                            const pharos_synth_row &row, const RowPrep *prep,
                                            const DevEngine &E, uint32_t n0, uint32_t tile_len, const uint8_t *masks,
                                            uint64_t mask_len, uint32_t *err, uint32_t j) {
@@ -351,7 +352,7 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
     if (first_lane) R.active = 0;
     if (row.table == PHAROS_SYNTH_INACTIVE) return;
 #pragma unroll
-    for (uint32_t b = j; b < TILE_BLOCKS; b += G) {
+    for (uint32_t b = j; b < TB; b += G) {
         double c, s;
         sincos64(row.carrier_phase0 + (double)(n0 + b * RESEED) * row.carrier_step, E.turn, c, s);
         R.c0_c[b] = c;
@@ -359,7 +360,7 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
     }
     if (row.table >= E.ntables) { if (first_lane) atomicOr(err, ERR_TABLE); return; }
     const auto &T = gmem(E.tables)[row.table];
-    const uint32_t per = T.per, w_max = T.w_max, ext_len = T.ext_len;
+    const uint32_t per = T.per, w_max = TB == TILE_BLOCKS ? T.w_max : T.w_max_wide, ext_len = T.ext_len;
     const uint32_t axis = T.axis, tdm = T.tdm_periods, count = T.count;
     const uint8_t *ext = T.ext;
     constexpr uint32_t PER = LANES16 / G;      // lane powers and value-table entries of each thread
@@ -478,7 +479,7 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
         const uint64_t block = (uint64_t)RESEED * eps_q;
         uint64_t pb = P0 + ((uint64_t)hi_base << 32) + (uint64_t)j * block;
 #pragma unroll
-        for (uint32_t bb = j; bb < TILE_BLOCKS; bb += G, pb += (uint64_t)G * block) R.pb[bb] = pb;
+        for (uint32_t bb = j; bb < TB; bb += G, pb += (uint64_t)G * block) R.pb[bb] = pb;
     }
     if (first_lane) {
         R.shape = shape;
@@ -499,7 +500,8 @@ __device__ __forceinline__ void chan_group(ChanRec &R, float2 *vt, uint32_t *sta
 // only static LDS variable, so it sits at LDS address 0 (synth_tile checks) and wave w's slot at the literal w * 128:
 // a value read takes the staged byte itself as its address, the slot going in the instruction's offset field, and
 // the per-sample address add disappears.
-__shared__ float2 g_wave_vt[THREADS / 64][16];
+__shared__ float2 g_wave_vt[TileGeo<TILE_BLOCKS>::WAVES][16];
+__shared__ float2 g_wave_vt_wide[TileGeo<WIDE_BLOCKS>::WAVES][16];  // the wide kernels' slots (four waves)
 constexpr uint32_t VT_SLOT_BYTES = 16 * sizeof(float2);
 
 // (mask & a) | (~mask & b) in one v_bfi_b32 (the compiler splits the expression into three or four operations);
@@ -628,8 +630,8 @@ __device__ __forceinline__ uint32_t bytes_for(uint32_t format, uint32_t samples)
 
 // Noise, quantisation and stores of a tile's sums (in LDS), over groups of contiguous samples: each group of G
 // samples quantises to one aligned unit of bytes (quantize.zig:16-135; noise kernel.zig:411-435).
-template <uint32_t F>
-__device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, uint32_t valid, uint64_t abs0, float gain,
+template <uint32_t F, uint32_t TB>
+__device__ __forceinline__ uint32_t epilogue(const Lds<TB> &L, const DevEngine &E, uint32_t valid, uint64_t abs0, float gain,
                                             uint8_t *dst) {
     constexpr uint32_t G = F == PHAROS_SYNTH_IQ2 ? 8u : 4u;
     constexpr uint32_t UB = F == PHAROS_SYNTH_IQ2 ? 4u : (F == PHAROS_SYNTH_F32 ? 32u : G << (F - 1));
@@ -728,7 +730,7 @@ __device__ __forceinline__ uint32_t epilogue(const Lds &L, const DevEngine &E, u
                 if (q < nb) d[q] = (uint8_t)(w[q >> 2] >> (8 * (q & 3)));
         }
     };
-    for (uint32_t g0 = threadIdx.x * G; g0 < valid; g0 += THREADS * G) {
+    for (uint32_t g0 = threadIdx.x * G; g0 < valid; g0 += TileGeo<TB>::THREADS * G) {
         if (g0 + G <= valid) group(g0, std::true_type{});
         else group(g0, std::false_type{});
     }
@@ -752,7 +754,7 @@ __device__ __forceinline__ DevEngine load_engine(const DevEngine *p) {
     e.chunk = g.chunk;
     e.lds_bytes = g.lds_bytes;
     e.out_bytes = g.out_bytes;
-    e.pad = 0;
+    e.wide = g.wide;
     return e;
 }
 
@@ -775,10 +777,14 @@ struct TileArgs {
 // access (the persistent kernel takes its next item there, so the queue's atomic returns while the sums run);
 // AfterSums by every thread when the last chunk's sums are done (the persistent kernel resolves that item there);
 // GainFn by every thread before quantising (the persistent kernel waits there for the AGC of the previous window).
-// Returns the clipped components of the tile.
-template <uint32_t F, class GainFn, class BeforeLoop, class AfterSums>
+// TB is the tile's rotor blocks (TILE_BLOCKS or WIDE_BLOCKS, with the engine view of that size).  Returns the
+// clipped components of the tile.
+template <uint32_t F, uint32_t TB, class GainFn, class BeforeLoop, class AfterSums>
 __device__ __forceinline__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem, GainFn gain_fn, BeforeLoop before_loop,
                                AfterSums after_sums, uint32_t *err) {
+    using Geo = TileGeo<TB>;
+    constexpr uint32_t THREADS = Geo::THREADS, TILE = Geo::TILE;
+    constexpr uint32_t GROUP = THREADS / MAX_CHUNK;  // threads per channel of a large pass
     DevEngine E = load_engine(A.E);
     if (A.chunk_cap && A.chunk_cap < E.chunk) E.chunk = A.chunk_cap;
     // Passes of equal size: as few passes as the pass size allows, the channels split evenly over them.
@@ -793,9 +799,13 @@ __device__ __forceinline__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem,
     const uint32_t tile_len = remain >= TILE ? TILE : ((remain + RESEED - 1) / RESEED) * RESEED;
     const uint32_t nb = n0 + b * RESEED;
     const bool block_live = nb < E.spe;
-    Lds L = lds_layout(smem, E);
+    Lds<TB> L = lds_layout<TB>(smem, E);
     const uint32_t smem_lds = lds_address(smem);
-    if (tid == 0 && lds_address(g_wave_vt) != 0) atomicOr(err, ERR_LAYOUT);
+    if constexpr (TB == TILE_BLOCKS) {
+        if (tid == 0 && lds_address(g_wave_vt) != 0) atomicOr(err, ERR_LAYOUT);
+    } else {
+        if (tid == 0 && lds_address(g_wave_vt_wide) != 0) atomicOr(err, ERR_LAYOUT);
+    }
 
     float ar[STEPS], ai[STEPS];
 #pragma unroll
@@ -812,17 +822,17 @@ __device__ __forceinline__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem,
     for (uint32_t c0 = 0; c0 < A.channels; c0 += E.chunk) {
         const uint32_t nc = min(E.chunk, A.channels - c0);
         // Setup and staging: one group of threads per channel of the chunk.
-        if (nc <= THREADS / 8) {
-            const uint32_t c = tid / 8, j = tid % 8;
+        if (nc <= THREADS / (2 * GROUP)) {
+            const uint32_t c = tid / (2 * GROUP), j = tid % (2 * GROUP);
             if (c < nc)
-                chan_group<8>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
+                chan_group<2 * GROUP, TB>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
                               reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot),
                               smem_lds + L.stage_base + c * E.w_slot, L.rows[c], A.prep ? A.prep + c0 + c : nullptr,
                               E, n0, tile_len, A.masks, A.mask_len, err, j);
         } else {
             const uint32_t c = tid / GROUP, j = tid % GROUP;
             if (c < nc)
-                chan_group<GROUP>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
+                chan_group<GROUP, TB>(L.rec[c], reinterpret_cast<float2 *>(L.vt + c * VT_BYTES),
                                   reinterpret_cast<uint32_t *>(L.stage + c * E.w_slot),
                                   smem_lds + L.stage_base + c * E.w_slot, L.rows[c],
                                   A.prep ? A.prep + c0 + c : nullptr, E, n0, tile_len, A.masks, A.mask_len, err, j);
@@ -841,7 +851,7 @@ __device__ __forceinline__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem,
             for (uint32_t ch = 0; ch < nc; ch++) {
                 // Every field of the channel's record and its value-table dword are read in one LDS round trip,
                 // ahead of the first branch on them.
-                const ChanRec &R = L.rec[ch];
+                const ChanRecT<TB> &R = L.rec[ch];
                 const uint32_t vt_word = reinterpret_cast<const uint32_t *>(L.vt + ch * VT_BYTES)[tid & 31u];
                 const double pc = R.pw_c[l], ps = R.pw_s[l], cc = R.c0_c[b], cs = R.c0_s[b];
                 const uint64_t pl = R.pl[l], pb = R.pb[b], step = R.step;
@@ -851,7 +861,8 @@ __device__ __forceinline__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem,
                 // This channel's values into the wave's slot (a dword per lane; lanes l and l + 32 write the same
                 // word with the same value): the LDS runs one wave's operations in order, so the value reads that
                 // follow see the writes.
-                reinterpret_cast<uint32_t *>(g_wave_vt[wave])[tid & 31u] = vt_word;
+                if constexpr (TB == TILE_BLOCKS) reinterpret_cast<uint32_t *>(g_wave_vt[wave])[tid & 31u] = vt_word;
+                else reinterpret_cast<uint32_t *>(g_wave_vt_wide[wave])[tid & 31u] = vt_word;
                 // The record's values are taken here, ahead of the test of `active`, so that their loads are not
                 // sunk past that branch into a second round trip (the statement emits no instruction).
                 asm volatile("" ::"v"(pc), "v"(ps), "v"(cc), "v"(cs), "v"(pl), "v"(pb), "v"(step), "v"(lead_whole),
@@ -864,7 +875,9 @@ __device__ __forceinline__ uint32_t synth_tile(const TileArgs &A, uint8_t *smem,
                     lead_dispatch(axis, ar, ai, pos, step, cre, cim, w_re, w_im,
                                   ((uint64_t)lead_whole << 32) + lead_q, lead_bits, wave * VT_SLOT_BYTES);
                 else if (wave == 0) channel_dispatch<0>(axis, ar, ai, pos, step, cre, cim, w_re, w_im);
-                else channel_dispatch<1>(axis, ar, ai, pos, step, cre, cim, w_re, w_im);
+                else if (TB == TILE_BLOCKS || wave == 1) channel_dispatch<1>(axis, ar, ai, pos, step, cre, cim, w_re, w_im);
+                else if (wave == 2) channel_dispatch<2>(axis, ar, ai, pos, step, cre, cim, w_re, w_im);
+                else channel_dispatch<3>(axis, ar, ai, pos, step, cre, cim, w_re, w_im);
             }
         }
         __syncthreads();

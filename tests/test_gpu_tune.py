@@ -8,8 +8,8 @@ from pharos_amd.engine import Job, run_batch
 pytestmark = pytest.mark.gpu
 
 
-def _run(scs, w, c):
-    jobs = run_batch([Job(s) for s in scs], workgroups_per_cu=w, channels_per_pass=c)
+def _run(scs, w, c, flags=0):
+    jobs = run_batch([Job(s) for s in scs], workgroups_per_cu=w, channels_per_pass=c, flags=flags)
     return [(j.result(), j.clipped.value, j.adjustments(), j.agc.gain) for j in jobs]
 
 
@@ -22,6 +22,34 @@ def test_launch_shapes_give_identical_results():
         for (rb, rc, ra, rg), (gb, gc, ga, gg) in zip(ref, got):
             assert np.array_equal(rb, gb), (w, c)
             assert rc == gc and ra == ga and rg == gg, (w, c)
+
+
+WIDE = 2  # PHAROS_SYNTH_TUNE_WIDE_TILES
+
+
+@pytest.mark.parametrize("case", ["multi_gnss", "wideband", "uneven_above_4096", "uneven_with_l1ca"])
+def test_wide_tiles_give_identical_results(case):
+    """8192-sample work items of 256 threads give the bytes, clips and AGC trace of 4096-sample ones: the multi-GNSS
+    run (7.5 wide tiles per epoch, so each epoch ends on a half-empty one), the wideband run, uneven jobs above 4096
+    samples per epoch (30.69 Msps: the last rotor block of an epoch is partial), and a batch holding 4.092 Msps jobs,
+    which has no wide form and runs narrow."""
+    if case == "multi_gnss":
+        scs = [S.c1(seed=4, epochs=160)]
+    elif case == "wideband":
+        scs = [S.c2(seed=4, epochs=120)]
+    else:
+        pool = S.c3(seed=11, count=24, epochs_cap=130)
+        scs = [s for s in pool if s.samples_per_epoch > 4096][:5]
+        if case == "uneven_with_l1ca":
+            scs = scs[:3] + [s for s in pool if s.samples_per_epoch <= 4096][:2]
+        assert any(s.samples_per_epoch == 30690 for s in pool)
+    print("case wide tiles", case, [(s.samples_per_epoch, s.format, len(s.channels), s.epochs) for s in scs], flush=True)
+    ref = _run(scs, 0, 0)
+    for w, c in [(0, 0), (2, 32), (1, 16), (2, 8)]:
+        got = _run(scs, w, c, WIDE)
+        for (rb, rc, ra, rg), (gb, gc, ga, gg) in zip(ref, got):
+            assert np.array_equal(rb, gb), (case, w, c)
+            assert rc == gc and ra == ga and rg == gg, (case, w, c)
 
 
 def test_tuner_picks_a_candidate(tmp_path):

@@ -14,9 +14,19 @@ def test_shape_key_buckets_channels_and_jobs():
 def test_cache_store_and_lookup(tmp_path):
     path = str(tmp_path / "tuning.json")
     sc = [S.c0(seed=1, epochs=10)]
-    assert tune.lookup(sc, path) == (0, 0)
-    tune.store(path, tune.shape_key(sc), (3, 12), 1.5)
-    assert tune.lookup(sc, path) == (3, 12)
-    tune.store(path, "other", (1, 4), 2.0)
-    assert tune.lookup(sc, path) == (3, 12)
-    assert len(tune.CANDIDATES) == 16
+    assert tune.lookup(sc, path) == (0, 0, 0)
+    tune.store(path, tune.shape_key(sc), (3, 12, 0), 1.5)
+    assert tune.lookup(sc, path) == (3, 12, 0)
+    tune.store(path, "other", (1, 4, tune.WIDE_TILES), 2.0)
+    assert tune.lookup(sc, path) == (3, 12, 0)
+    assert len(tune.CANDIDATES) == 24
+
+
+def test_wide_candidates_only_for_long_epochs():
+    """Wide tiles are timed only when every scenario's epoch is longer than a narrow tile: the library runs a batch
+    with any shorter epoch narrow."""
+    c0, c1 = S.c0(seed=1, epochs=10), S.c1(seed=1, epochs=10)
+    assert len(tune.candidates([c1])) == 24 and any(f for _, _, f in tune.candidates([c1]))
+    assert len(tune.candidates([c0])) == 16 and not any(f for _, _, f in tune.candidates([c0]))
+    assert not any(f for _, _, f in tune.candidates([c0, c1]))
+    assert tune.label(2, 32, tune.WIDE_TILES) == "2x32w" and tune.label(4, 24, 0) == "4x24"
