@@ -224,8 +224,9 @@ typedef struct pharos_synth_tuning {
  * the wait limit or pharos_synth_batch_abort ends the wait. */
 #define PHAROS_SYNTH_TUNE_HOLD_FIRST_GAIN 1u
 /* Wide tiles: 8192-sample work items of 256 threads (two workgroups per CU) instead of 4096-sample ones of 128, so a
- * tile's setup, barriers and gain wait serve twice the samples.  Taken when every job's engine has more than 4096
- * samples per epoch; the bytes are the same either way. */
+ * tile's setup, barriers and gain wait serve twice the samples.  Taken for the jobs whose engines have more than
+ * 4096 samples per epoch; the batch's other jobs run narrow in a second launch on the same stream.  The bytes are
+ * the same either way. */
 #define PHAROS_SYNTH_TUNE_WIDE_TILES 2u
 
 /* Runs all jobs to completion on `stream` and synchronises it.  Each job's bytes equal what the plain form

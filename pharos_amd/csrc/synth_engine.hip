@@ -1214,16 +1214,20 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
             off += n;
         }
     }
-    // One launch: a single-format batch runs the kernel instance with only that format's quantiser, a mixed one
-    // the instance that picks each job's quantiser per tile.
-    // Wide tiles when the launch shape asks for them and every job's engine has them.
-    bool wide = tuning && (tuning->flags & PHAROS_SYNTH_TUNE_WIDE_TILES);
-    for (uint32_t j = 0; j < njobs && wide; j++) wide = jobs[j].engine->view_wide.tiles != 0;
-    auto view_of = [&](const pharos_synth_engine *e) -> const DevEngine & { return wide ? e->view_wide : e->view; };
-    uint32_t batch_format = jobs[0].engine->view.format, batch_lds = 0;
+    // Wide tiles, when the launch shape asks for them, for every job whose engine has them; the others run narrow.
+    // Each tile size is one launch (wide jobs' items first, then the narrow ones', on the same stream).  A launch
+    // of one format runs the kernel instance with only that format's quantiser, a mixed one the instance that picks
+    // each job's quantiser per tile.
+    const bool want_wide = tuning && (tuning->flags & PHAROS_SYNTH_TUNE_WIDE_TILES);
+    std::vector<uint8_t> jw(njobs, 0);
+    for (uint32_t j = 0; j < njobs; j++) jw[j] = want_wide && jobs[j].engine->view_wide.tiles != 0;
+    auto view_of = [&](uint32_t j) -> const DevEngine & { return jw[j] ? jobs[j].engine->view_wide : jobs[j].engine->view; };
+    uint32_t launch_format[2] = {0xffffffffu, 0xffffffffu}, launch_lds[2] = {0, 0}, launch_jobs[2] = {0, 0};
     for (uint32_t j = 0; j < njobs; j++) {
-        if (jobs[j].engine->view.format != batch_format) batch_format = FORMAT_ANY;
-        batch_lds = std::max(batch_lds, view_of(jobs[j].engine).lds_bytes);
+        const uint32_t g = jw[j] ? 0 : 1, f = jobs[j].engine->view.format;
+        launch_format[g] = launch_format[g] == 0xffffffffu || launch_format[g] == f ? f : FORMAT_ANY;
+        launch_lds[g] = std::max(launch_lds[g], view_of(j).lds_bytes);
+        launch_jobs[g]++;
     }
     // Queue order: window w of a job of W windows at (w + 1/2) / W of the launch, so every job's windows spread over
     // the whole launch and the jobs end together.  (Window by window across jobs, a job with many short windows ran
@@ -1234,7 +1238,8 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     for (uint32_t j = 0; j < njobs; j++)
         for (uint32_t w = 0; w < (uint32_t)per_job[j].size(); w++)
             order.push_back({2ull * w + 1, 2ull * per_job[j].size(), j, w});
-    std::stable_sort(order.begin(), order.end(), [](const Slot &a, const Slot &b) {
+    std::stable_sort(order.begin(), order.end(), [&](const Slot &a, const Slot &b) {
+        if (jw[a.job] != jw[b.job]) return jw[a.job] > jw[b.job];  // the wide launch's items first
         const unsigned __int128 l = (unsigned __int128)a.num * b.den, r = (unsigned __int128)b.num * a.den;
         return l != r ? l < r : (a.job != b.job ? a.job < b.job : a.win < b.win);
     });
@@ -1243,7 +1248,7 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     std::vector<uint32_t> last_seg(njobs, 0xffffffffu);
     h->job_first_seg.assign(njobs, 0xffffffffu);
     h->job_seg_count.assign(njobs, 0);
-    uint64_t items = 0;
+    uint64_t items = 0, items_wide = 0;
     for (const Slot &o : order) {
         const uint32_t j = o.job;
         const Win &w = per_job[j][o.win];
@@ -1252,8 +1257,9 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
         s.job = j;
         s.epoch_off = w.off;
         s.epochs = w.epochs;
-        s.tiles = view_of(jobs[j].engine).tiles;
+        s.tiles = view_of(j).tiles;
         items += (uint64_t)s.epochs * s.tiles;
+        if (jw[j]) items_wide = items;
         const uint32_t idx = (uint32_t)segs.size();
         segs.push_back(s);
         h->seg_job.push_back(j);
@@ -1278,7 +1284,7 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
         const pharos_synth_job &J = jobs[j];
         DevJob &d = dj[j];
         memset(&d, 0, sizeof d);
-        d.engine = wide ? J.engine->d_engine_wide : J.engine->d_engine;
+        d.engine = jw[j] ? J.engine->d_engine_wide : J.engine->d_engine;
         d.rows = J.rows;
         d.masks = J.masks;
         d.out = J.out;
@@ -1305,7 +1311,7 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     size_t total = 0;
     auto carve = [&](size_t n) { size_t o = (total + 255) & ~(size_t)255; total = o + n; return Piece{o, n}; };
     Piece p_jobs = carve(sizeof(DevJob) * njobs), p_segs = carve(sizeof(DevSegment) * nsegs),
-          p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(sizeof(unsigned long long)),
+          p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(2 * sizeof(unsigned long long)),
           p_gain = carve(sizeof(unsigned long long) * nsegs), p_sob = carve(sizeof(uint32_t) * (items / SEG_BLOCK + 1)),
           p_agc = carve(sizeof(DevAgc) * njobs), p_count = carve(sizeof(unsigned long long) * nsegs),
           p_err = carve(64);
@@ -1316,7 +1322,7 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     std::vector<size_t> prep_at(njobs, SIZE_MAX);
     size_t prep_rows = 0;
     for (uint32_t j = 0; j < njobs; j++) {
-        if (view_of(jobs[j].engine).tiles < 2) continue;
+        if (view_of(j).tiles < 2) continue;
         prep_at[j] = prep_rows;
         prep_rows += (size_t)jobs[j].epochs * jobs[j].channels;
     }
@@ -1331,6 +1337,8 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     memcpy(host + p_jobs.off, dj.data(), p_jobs.size);
     memcpy(host + p_segs.off, segs.data(), p_segs.size);
     memcpy(host + p_next.off, seg_next.data(), p_next.size);
+    // The work queues: the wide launch's next item starts at 0, the narrow launch's after the wide items.
+    memcpy(host + p_queue.off + sizeof(unsigned long long), &items_wide, sizeof items_wide);
     for (uint32_t k = 0; k < nsegs; k++) {
         uint32_t bits;
         memcpy(&bits, &seg_gain[k], sizeof bits);
@@ -1377,7 +1385,6 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     S.wait_limit = tuning && tuning->wait_limit ? tuning->wait_limit : (1u << 24);
     h->abort_slot = abort_take();
     S.abort_word = h->abort_slot >= 0 ? g_abort_dev + h->abort_slot : nullptr;
-    S.queue_words_offset = (batch_lds + 15u) & ~15u;
     S.chunk_cap = tuning ? std::min<uint32_t>(tuning->channels_per_pass, MAX_CHUNK) : 0;
     h->d_agc = S.agc;
     h->d_adjust = S.adjust;
@@ -1397,12 +1404,21 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
             return ST_INTERNAL;
         }
     }
-    {
-        const uint32_t lds = S.queue_words_offset + 16;
+    // The launches: wide jobs' items [0, items_wide), then narrow jobs' items [items_wide, items), each a
+    // persistent grid with its own work queue.  A split batch's narrow launch takes the default occupancy; the
+    // launch shape's workgroups per CU is for the tile size it was tuned with.
+    for (uint32_t g = 0; g < 2; g++) {
+        if (!launch_jobs[g]) continue;
+        const bool wide = g == 0;
+        BatchState L = S;
+        L.queue = S.queue + g;
+        L.total_items = wide ? items_wide : items;
+        L.queue_words_offset = (launch_lds[g] + 15u) & ~15u;
+        const uint32_t lds = L.queue_words_offset + 16;
         const void *fn = nullptr;
 #define PHAROS_BATCH_KERNEL(FMT) \
     (wide ? (const void *)synth_batch_kernel<FMT, WIDE_BLOCKS> : (const void *)synth_batch_kernel<FMT, TILE_BLOCKS>)
-        switch (batch_format) {
+        switch (launch_format[g]) {
         case PHAROS_SYNTH_IQ2: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ2); break;
         case PHAROS_SYNTH_IQ4: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ4); break;
         case PHAROS_SYNTH_IQ8: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ8); break;
@@ -1412,17 +1428,19 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
         }
 #undef PHAROS_BATCH_KERNEL
         const uint32_t threads = wide ? TileGeo<WIDE_BLOCKS>::THREADS : THREADS;
-        uint32_t per_cu = tuning && tuning->workgroups_per_cu ? tuning->workgroups_per_cu : 0;
+        const bool split = launch_jobs[0] && launch_jobs[1];
+        uint32_t per_cu = tuning && tuning->workgroups_per_cu && (wide || !split) ? tuning->workgroups_per_cu : 0;
         if (!per_cu) {
             int occ = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, lds) != hipSuccess || occ <= 0) occ = wide ? 1 : 2;
             per_cu = (uint32_t)occ;
         }
         if (wide) per_cu = std::min<uint32_t>(per_cu, 2);  // a wide workgroup takes half a CU's LDS at most
+        const uint64_t launch_items = wide ? items_wide : items - items_wide;
         uint64_t grid = (uint64_t)cu_count(device) * per_cu;
-        if (grid > items) grid = items;
+        if (grid > launch_items) grid = launch_items;
         if (grid == 0) grid = 1;
-        void *args[] = {&S};
+        void *args[] = {&L};
         if (hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(threads), args, lds, h->stream) != hipSuccess) {
             set_error("batch launch failed");
             (void)hipStreamSynchronize(h->stream);

@@ -1,8 +1,8 @@
 """The engine's own autotuner for the batched launch shape.
 
 The tunable knobs are those of pharos_synth_tuning:
-- the tile: 4096 samples on 128 threads, or 8192 on 256 (PHAROS_SYNTH_TUNE_WIDE_TILES), for workloads whose epochs
-  are longer than one narrow tile;
+- the tile: 4096 samples on 128 threads, or 8192 on 256 (PHAROS_SYNTH_TUNE_WIDE_TILES) for the jobs whose epochs
+  are longer than one narrow tile (the others then run narrow in a second launch);
 - persistent workgroups per compute unit (1-4 narrow, 1-2 wide: a tile's LDS allows at most that many);
 - channels staged per LDS pass of a tile (8-32).
 
@@ -34,9 +34,9 @@ def label(w, c, flags):
 
 
 def candidates(scenarios):
-    """The shapes worth timing: wide tiles only when every scenario's epoch is longer than a narrow tile (the library
-    runs a batch narrow otherwise)."""
-    wide = all(s.samples_per_epoch > TILE for s in scenarios)
+    """The shapes worth timing: wide tiles only when some scenario's epoch is longer than a narrow tile (the library
+    runs those jobs wide and the others narrow, in a second launch)."""
+    wide = any(s.samples_per_epoch > TILE for s in scenarios)
     return NARROW + (WIDE if wide else [])
 
 
