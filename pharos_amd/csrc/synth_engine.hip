@@ -388,7 +388,7 @@ __device__ __forceinline__ uint32_t wait_gain(const BatchState &S, uint32_t s) {
 }
 
 template <uint32_t F, uint32_t TB>
-__global__ __launch_bounds__(TileGeo<TB>::THREADS, TB == TILE_BLOCKS ? 2 : 1) void synth_batch_kernel(BatchState S) {
+__global__ __launch_bounds__(TileGeo<TB>::THREADS, 2) void synth_batch_kernel(BatchState S) {
     constexpr uint32_t THREADS = TileGeo<TB>::THREADS;
     extern __shared__ __align__(16) uint8_t smem[];
     // Queue words live after the largest layout of any job (synth_tile's reduction words sit below them).

@@ -496,10 +496,10 @@ __device__ __forceinline__ void chan_group(ChanRecT<TB> &R, float2 *vt, uint32_t
 
 // The per-sample loop of one channel for one thread: 32 samples at stride 16 of one rotor block.
 // pos: high word = LDS address of the staged entry byte, low word = Q32.32 fraction of the code position.
-// The value table of the channel each wave is summing, copied to a fixed LDS slot per wave.  It is the kernels'
-// only static LDS variable, so it sits at LDS address 0 (synth_tile checks) and wave w's slot at the literal w * 128:
-// a value read takes the staged byte itself as its address, the slot going in the instruction's offset field, and
-// the per-sample address add disappears.
+// The value table of the channel each wave is summing, copied to a fixed LDS slot per wave.  A kernel uses one of
+// the two slot arrays (narrow tiles: two waves, wide tiles: four) as its only static LDS variable, so it sits at LDS
+// address 0 (synth_tile checks) and wave w's slot at the literal w * 128: a value read takes the staged byte itself
+// as its address, the slot going in the instruction's offset field, and the per-sample address add disappears.
 __shared__ float2 g_wave_vt[TileGeo<TILE_BLOCKS>::WAVES][16];
 __shared__ float2 g_wave_vt_wide[TileGeo<WIDE_BLOCKS>::WAVES][16];  // the wide kernels' slots (four waves)
 constexpr uint32_t VT_SLOT_BYTES = 16 * sizeof(float2);
