@@ -269,8 +269,22 @@ pharos_run::Prepared prepare(const pharos_run *r, uint64_t bstart) {
     return p;
 }
 
-// The level steps to the device when they changed since the last upload.
-int upload_levels(pharos_run *r) {
+// The level steps to the device when they changed since the last upload.  Rows are derived from block starts
+// onwards and a run only moves forward (a seek backwards restarts it), so of each channel's steps at or before
+// `from` only the last can still reach a row: the earlier ones are dropped, which bounds the steps by the commands
+// of the current block.
+int upload_levels(pharos_run *r, uint64_t from) {
+    std::vector<pharos_synth_level> &v = r->levels;
+    size_t w = 0;
+    for (size_t i = 0; i < v.size(); i++) {
+        const bool superseded = i + 1 < v.size() && v[i + 1].channel == v[i].channel && v[i + 1].epoch <= from;
+        if (superseded) {
+            r->levels_dirty = true;
+            continue;
+        }
+        v[w++] = v[i];
+    }
+    v.resize(w);
     if (!r->levels_dirty) return 0;
     const uint32_t n = (uint32_t)r->levels.size();
     if (n > r->d_levels_cap) {
@@ -319,7 +333,7 @@ int synthesise(pharos_run *r, uint64_t first, uint32_t count, uint8_t *out, uint
         } catch (...) {  // no thread to be had: the next block is prepared when it is reached
         }
     }
-    if (!st) st = upload_levels(r);
+    if (!st) st = upload_levels(r, bstart);
     if (!st && nch)
         st = pharos_synth_derive_levels((const pharos_synth_segment *)r->d_segs, blen, bstart, skip + count, nch,
                                         r->spec.start_gps_ms, r->spec.start_gps_sub_ms, r->spec.samples_per_epoch,

@@ -107,6 +107,43 @@ def test_commands_apply_at_their_epoch():
     assert run.seek(0) == STATUS["arg"] and "applied" in run.error()
 
 
+def test_successive_levels_of_one_satellite():
+    """Three commands on one satellite in three blocks (31 dB-Hz at 150, 45 at 260, its default again at 380): each
+    stretch equals the scenario with that satellite at that level from the start, so the device's level steps keep
+    every command's epoch while the older steps of a channel are dropped as the run moves on."""
+    sc = small(gain_db=0.0, epochs=520)
+    sat = sc.channels[3]
+    system = {"L1CA": 0, "L1C": 0, "B1I": 1, "E1": 2, "G1": 3}[sat.signal]
+    print("case levels", sat.signal, sat.prn, flush=True)
+    run = Run(sc.bundle(), seed=sc.seed, gain=1, gain_db=0.0)
+    assert run.submit(2, value=31.0, apply_at_epoch=150, system=system, svid=sat.prn, unit=0) == STATUS["ok"]
+    assert run.submit(2, value=45.0, apply_at_epoch=260, system=system, svid=sat.prn, unit=0) == STATUS["ok"]
+    assert run.submit(2, apply_at_epoch=380, system=system, svid=sat.prn, use_default=1) == STATUS["ok"]
+    data = run.pull_all(70)
+    bpe = sc.bytes_per_epoch
+    assert [(k, e) for k, e, _, _, _, _ in run.events()] == [(2, 150), (2, 260), (2, 380)]
+
+    def at_level(cn0, first, last):
+        for c in sc.channels:
+            if c.signal == sat.signal and c.prn == sat.prn:
+                saved, c.cn0_dbhz = c.cn0_dbhz, cn0
+        other = Run(sc.bundle(), seed=sc.seed, gain=1, gain_db=0.0)
+        for c in sc.channels:
+            if c.signal == sat.signal and c.prn == sat.prn:
+                c.cn0_dbhz = saved
+        assert other.seek(first) == STATUS["ok"]
+        st, _, got, _ = other.pull(last - first)
+        assert st == STATUS["ok"]
+        return got
+
+    base = Run(sc.bundle(), seed=sc.seed, gain=1, gain_db=0.0).pull_all(520)
+    assert np.array_equal(data[:150 * bpe], base[:150 * bpe])
+    assert np.array_equal(data[150 * bpe:260 * bpe], at_level(31.0, 150, 260))
+    assert np.array_equal(data[260 * bpe:380 * bpe], at_level(45.0, 260, 380))
+    assert np.array_equal(data[380 * bpe:], base[380 * bpe:])
+    assert not np.array_equal(data[150 * bpe:380 * bpe], base[150 * bpe:380 * bpe])
+
+
 def test_submit_rules():
     sc = small(gain_db=0.0, epochs=300)
     run = Run(sc.bundle(), seed=sc.seed, gain=1, gain_db=0.0)
