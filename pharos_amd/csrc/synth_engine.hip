@@ -461,9 +461,14 @@ __global__ __launch_bounds__(TileGeo<TB>::THREADS, 2) void synth_batch_kernel(Ba
         const uint32_t clips = synth_tile<F, TB>(A, smem, gain_fn, before_loop, after_sums, S.err);
         if (threadIdx.x == 0) {
             // One atomic counts the tile and its clips.  Under the adaptive gain the last tile of the segment gets
-            // the segment's total and runs the AGC step; under a constant gain nothing waits for the count.
+            // the segment's total and runs the AGC step; under a constant gain nothing waits for the count, and the
+            // atomic takes the form that returns nothing.  A returning one leaves its result register pending, so
+            // the next item's first write of that register waits on the memory counter, which also covers every
+            // load and store issued since (1 % on the 8-channel L1 C/A run).  The empty statement keeps the compiler
+            // from merging the two branches' atomics into one returning atomic ahead of the branch.
             const unsigned long long add = (1ull << COUNT_SHIFT) | clips;
             if (adaptive) {
+                asm volatile("");
                 const unsigned long long now =
                     __hip_atomic_fetch_add(&S.seg_count[s], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + add;
                 if ((now >> COUNT_SHIFT) == (unsigned long long)seg.epochs * seg.tiles)
