@@ -2,8 +2,6 @@
 import numpy as np
 
 from oracle import binding as O
-from pharos_amd import _lib
-from pharos_amd.scenario import Scenario, Channel
 
 F32 = 4
 

@@ -6,7 +6,7 @@ import pytest
 
 from oracle import binding as O
 from pharos_amd import _lib, scenario as S
-from pharos_amd.engine import DeviceBuffer, Job, derive_rows, run_batch
+from pharos_amd.engine import Job, derive_rows, run_batch
 from pharos_amd.scenario import Channel, Scenario
 from tests.helpers import oracle_window
 

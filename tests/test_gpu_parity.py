@@ -8,7 +8,6 @@ import numpy as np
 import pytest
 
 from pharos_amd import scenario as S
-from pharos_amd.scenario import Scenario, Channel
 from tests.helpers import oracle_window, gpu_window, describe_mismatch
 
 pytestmark = pytest.mark.gpu

@@ -5,7 +5,6 @@ import ctypes as C
 import datetime as dt
 
 import numpy as np
-import pytest
 
 from pharos_amd import _lib
 
