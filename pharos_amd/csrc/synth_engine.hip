@@ -101,8 +101,10 @@ struct BatchState {
     const DevSegment *segs;
     const uint32_t *seg_next;      // the same job's next segment, or ~0
     uint32_t nsegs;
-    uint64_t total_items;          // the launch's work items are 0 .. total_items - 1
+    uint64_t first_item;           // the launch's work items are first_item .. total_items - 1
+    uint64_t total_items;
     unsigned long long *queue;     // next work item
+    uint32_t direct_first;         // 1: workgroup b starts on first_item + b, the queue past the grid
     // Per segment, two 64-bit words, each read and written only as a whole with relaxed device-scope atomics,
     // so the per-item path needs no fence:
     //   seg_count: finished tiles << COUNT_SHIFT | clipped components of those tiles
@@ -397,10 +399,15 @@ __global__ __launch_bounds__(TileGeo<TB>::THREADS, 2) void synth_batch_kernel(Ba
     lds_word *q = (lds_word *)(uintptr_t)lds_address(smem + S.queue_words_offset);
     // The work queue is the last thread's: it takes the next item once a tile's channel loop starts and resolves its
     // segment once the sums are done, so the queue's atomic and the segment search stay off the other threads' path.
+    // The first item: when no job of the launch waits for a gain, no item waits on another, and workgroup b takes
+    // item first_item + b without the queue (the whole grid taking its first item from the one queue word at once
+    // cost each workgroup about 3 us).  Under gain waits every item is taken from the queue, in order, by a
+    // running workgroup, so the earliest unfinished item never waits on an item that no running workgroup holds,
+    // whatever else shares the device.
     constexpr uint32_t QT = THREADS - 1;
     uint32_t hint = 0;  // the queue thread's segment cursor
     if (threadIdx.x == QT) {
-        const unsigned long long item = atomicAdd(S.queue, 1ull);
+        const unsigned long long item = S.direct_first ? S.first_item + blockIdx.x : atomicAdd(S.queue, 1ull);
         q[0] = (uint32_t)item;
         q[1] = (uint32_t)(item >> 32);
         if (item < S.total_items) {
@@ -1276,6 +1283,47 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     }
     const uint32_t nsegs = (uint32_t)segs.size();
     h->nsegs = nsegs;
+    // The launches: wide jobs' items [0, items_wide), then narrow jobs' items [items_wide, items), each a
+    // persistent grid with its own work queue.  A split batch's narrow launch takes the default occupancy; the
+    // launch shape's workgroups per CU is for the tile size it was tuned with.
+    struct Launch { const void *fn; uint32_t threads, queue_words_offset, lds, direct; uint64_t first, total, grid; };
+    Launch launch[2] = {};
+    bool waits[2] = {false, false};  // some job of the launch has the adaptive gain
+    for (uint32_t j = 0; j < njobs; j++)
+        if (jobs[j].gain_mode == PHAROS_SYNTH_GAIN_ADAPTIVE) waits[jw[j] ? 0 : 1] = true;
+    for (uint32_t g = 0; g < 2; g++) {
+        if (!launch_jobs[g]) continue;
+        const bool wide = g == 0;
+        Launch &LN = launch[g];
+        LN.first = wide ? 0 : items_wide;
+        LN.total = wide ? items_wide : items;
+        LN.queue_words_offset = (launch_lds[g] + 15u) & ~15u;
+        LN.lds = LN.queue_words_offset + 16;
+#define PHAROS_BATCH_KERNEL(FMT) \
+    (wide ? (const void *)synth_batch_kernel<FMT, WIDE_BLOCKS> : (const void *)synth_batch_kernel<FMT, TILE_BLOCKS>)
+        switch (launch_format[g]) {
+        case PHAROS_SYNTH_IQ2: LN.fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ2); break;
+        case PHAROS_SYNTH_IQ4: LN.fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ4); break;
+        case PHAROS_SYNTH_IQ8: LN.fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ8); break;
+        case PHAROS_SYNTH_IQ16: LN.fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ16); break;
+        case PHAROS_SYNTH_F32: LN.fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_F32); break;
+        default: LN.fn = PHAROS_BATCH_KERNEL(FORMAT_ANY); break;
+        }
+#undef PHAROS_BATCH_KERNEL
+        LN.threads = wide ? TileGeo<WIDE_BLOCKS>::THREADS : THREADS;
+        const bool split = launch_jobs[0] && launch_jobs[1];
+        uint32_t per_cu = tuning && tuning->workgroups_per_cu && (wide || !split) ? tuning->workgroups_per_cu : 0;
+        if (!per_cu) {
+            int occ = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, LN.fn, LN.threads, LN.lds) != hipSuccess || occ <= 0)
+                occ = wide ? 1 : 2;
+            per_cu = (uint32_t)occ;
+        }
+        if (wide) per_cu = std::min<uint32_t>(per_cu, 2);  // a wide workgroup takes half a CU's LDS at most
+        LN.grid = std::min<uint64_t>((uint64_t)cu_count(device) * per_cu, LN.total - LN.first);
+        if (LN.grid == 0) LN.grid = 1;
+        LN.direct = waits[g] ? 0 : 1;
+    }
     uint32_t adjust_cap = 1;
     for (uint32_t j = 0; j < njobs; j++) adjust_cap = std::max(adjust_cap, jobs[j].adjustments_cap);
     h->adjust_cap = adjust_cap;
@@ -1342,8 +1390,11 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     memcpy(host + p_jobs.off, dj.data(), p_jobs.size);
     memcpy(host + p_segs.off, segs.data(), p_segs.size);
     memcpy(host + p_next.off, seg_next.data(), p_next.size);
-    // The work queues: the wide launch's next item starts at 0, the narrow launch's after the wide items.
-    memcpy(host + p_queue.off + sizeof(unsigned long long), &items_wide, sizeof items_wide);
+    // The work queues: at the launch's first item, or past the grid when workgroup b starts on the first item + b.
+    for (uint32_t g = 0; g < 2; g++) {
+        const unsigned long long next = launch[g].first + (launch[g].direct ? launch[g].grid : 0);
+        memcpy(host + p_queue.off + g * sizeof(unsigned long long), &next, sizeof next);
+    }
     for (uint32_t k = 0; k < nsegs; k++) {
         uint32_t bits;
         memcpy(&bits, &seg_gain[k], sizeof bits);
@@ -1377,8 +1428,10 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     S.segs = (const DevSegment *)(b8 + p_segs.off);
     S.seg_next = (const uint32_t *)(b8 + p_next.off);
     S.nsegs = nsegs;
+    S.first_item = 0;
+    S.direct_first = 0;
     S.total_items = items;
-    S.queue = (unsigned long long *)(b8 + p_queue.off);  // the next item; starts at 0
+    S.queue = (unsigned long long *)(b8 + p_queue.off);  // one word per launch
     S.seg_count = (unsigned long long *)(b8 + p_count.off);
     S.seg_gain = (unsigned long long *)(b8 + p_gain.off);
     S.agc = (DevAgc *)(b8 + p_agc.off);
@@ -1409,44 +1462,17 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
             return ST_INTERNAL;
         }
     }
-    // The launches: wide jobs' items [0, items_wide), then narrow jobs' items [items_wide, items), each a
-    // persistent grid with its own work queue.  A split batch's narrow launch takes the default occupancy; the
-    // launch shape's workgroups per CU is for the tile size it was tuned with.
     for (uint32_t g = 0; g < 2; g++) {
         if (!launch_jobs[g]) continue;
-        const bool wide = g == 0;
+        const Launch &LN = launch[g];
         BatchState L = S;
         L.queue = S.queue + g;
-        L.total_items = wide ? items_wide : items;
-        L.queue_words_offset = (launch_lds[g] + 15u) & ~15u;
-        const uint32_t lds = L.queue_words_offset + 16;
-        const void *fn = nullptr;
-#define PHAROS_BATCH_KERNEL(FMT) \
-    (wide ? (const void *)synth_batch_kernel<FMT, WIDE_BLOCKS> : (const void *)synth_batch_kernel<FMT, TILE_BLOCKS>)
-        switch (launch_format[g]) {
-        case PHAROS_SYNTH_IQ2: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ2); break;
-        case PHAROS_SYNTH_IQ4: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ4); break;
-        case PHAROS_SYNTH_IQ8: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ8); break;
-        case PHAROS_SYNTH_IQ16: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_IQ16); break;
-        case PHAROS_SYNTH_F32: fn = PHAROS_BATCH_KERNEL(PHAROS_SYNTH_F32); break;
-        default: fn = PHAROS_BATCH_KERNEL(FORMAT_ANY); break;
-        }
-#undef PHAROS_BATCH_KERNEL
-        const uint32_t threads = wide ? TileGeo<WIDE_BLOCKS>::THREADS : THREADS;
-        const bool split = launch_jobs[0] && launch_jobs[1];
-        uint32_t per_cu = tuning && tuning->workgroups_per_cu && (wide || !split) ? tuning->workgroups_per_cu : 0;
-        if (!per_cu) {
-            int occ = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, lds) != hipSuccess || occ <= 0) occ = wide ? 1 : 2;
-            per_cu = (uint32_t)occ;
-        }
-        if (wide) per_cu = std::min<uint32_t>(per_cu, 2);  // a wide workgroup takes half a CU's LDS at most
-        const uint64_t launch_items = wide ? items_wide : items - items_wide;
-        uint64_t grid = (uint64_t)cu_count(device) * per_cu;
-        if (grid > launch_items) grid = launch_items;
-        if (grid == 0) grid = 1;
+        L.first_item = LN.first;
+        L.total_items = LN.total;
+        L.direct_first = LN.direct;
+        L.queue_words_offset = LN.queue_words_offset;
         void *args[] = {&L};
-        if (hipLaunchKernel(fn, dim3((uint32_t)grid), dim3(threads), args, lds, h->stream) != hipSuccess) {
+        if (hipLaunchKernel(LN.fn, dim3((uint32_t)LN.grid), dim3(LN.threads), args, LN.lds, h->stream) != hipSuccess) {
             set_error("batch launch failed");
             (void)hipStreamSynchronize(h->stream);
             batch_free(h);

@@ -3,6 +3,8 @@
 Each job's bytes, clip count and AGC adjustments must be the same alone and in any mix (the reference's
 determinism contract, docs/design/engine.md:235-243), and the device AGC must follow gain.zig:19-35 exactly.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -11,6 +13,8 @@ from pharos_amd.scenario import Scenario, Channel
 from pharos_amd.engine import Job, run_batch, run_plain
 
 pytestmark = pytest.mark.gpu
+
+WIDE_TILES = 2  # PHAROS_SYNTH_TUNE_WIDE_TILES
 
 
 def _strong_iq4(seed, spe=16_368, epochs=700, first=0):
@@ -51,6 +55,23 @@ def test_uneven_batch_each_job_as_alone():
         assert np.array_equal(j.result(), pb), s.name
         assert j.clipped.value == int(pc.sum()), s.name
         assert j.adjustments() == padj and j.agc.gain == pgain, s.name
+
+
+def test_constant_gain_batch_each_job_as_alone():
+    """With no gain waits each workgroup starts on the item of its own index and takes the rest from the queue: a
+    batch of many more items than workgroups, in one narrow launch and, with wide tiles asked for, split into a
+    wide and a narrow launch."""
+    jobs_sc = [dataclasses.replace(s, gain_db=-2.0) for s in S.c3(seed=4, count=5, epochs_cap=260)]
+    jobs_sc.append(S.c0(seed=12, epochs=300))
+    assert any(s.samples_per_epoch > 4096 for s in jobs_sc) and any(s.samples_per_epoch <= 4096 for s in jobs_sc)
+    plain = [run_plain(s) for s in jobs_sc]
+    for flags in (0, WIDE_TILES):
+        print("case constant-gain batch, flags", flags,
+              [(s.samples_per_epoch, s.format, len(s.channels), s.epochs) for s in jobs_sc], flush=True)
+        jobs = run_batch([Job(s) for s in jobs_sc], flags=flags)
+        for s, j, (pb, pc, _, _) in zip(jobs_sc, jobs, plain):
+            assert np.array_equal(j.result(), pb), (s.name, flags)
+            assert j.clipped.value == int(pc.sum()), (s.name, flags)
 
 
 def test_batch_from_a_mid_run_epoch():
