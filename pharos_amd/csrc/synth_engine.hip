@@ -125,21 +125,25 @@ struct BatchState {
 constexpr uint32_t SEG_BLOCK = 256;
 
 // The segment holding `item`: the cursor's segment or the next one, else from the table of every SEG_BLOCK-th
-// item's segment and a walk forward (segments hold their items in order, so a few steps at most; a binary search
-// over the rest if the walk runs long).
+// item's segment and a walk of up to four segments forward (segments hold their items in order), then a binary
+// search over the rest if the walk runs long.  Each step's loads are independent of one another and issue
+// together: a workgroup's first item finds its segment in three round trips on a cold cache instead of six.
 __device__ __forceinline__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
     const auto segs = gmem(S.segs);
-    if (hint < S.nsegs && segs[hint].first_item <= item &&
-        (hint + 1 == S.nsegs || segs[hint + 1].first_item > item))
-        return hint;
-    if (hint + 1 < S.nsegs && segs[hint + 1].first_item <= item &&
-        (hint + 2 == S.nsegs || segs[hint + 2].first_item > item))
-        return hint + 1;
-    uint32_t lo = gmem(S.seg_of_block)[item / SEG_BLOCK];
-    for (int k = 0; k < 4; k++) {
-        if (lo + 1 == S.nsegs || segs[lo + 1].first_item > item) return lo;
-        lo++;
+    auto first_of = [&](uint32_t k) -> uint64_t { return k < S.nsegs ? segs[k].first_item : ~0ull; };
+    {
+        const uint64_t f0 = first_of(hint), f1 = first_of(hint + 1), f2 = first_of(hint + 2);
+        if (f0 <= item && item < f1) return hint;
+        if (f1 <= item && item < f2) return hint + 1;
     }
+    uint32_t lo = gmem(S.seg_of_block)[item / SEG_BLOCK];  // first_item[lo] <= item
+    uint64_t f[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) f[k] = first_of(lo + 1 + k);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++)
+        if (item < f[k]) return lo + k;
+    lo += 4;
     uint32_t hi = S.nsegs;  // first_item[lo] <= item < first_item[hi]
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) / 2;
