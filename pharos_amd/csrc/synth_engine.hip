@@ -117,39 +117,19 @@ struct BatchState {
     uint32_t *err;
     uint32_t queue_words_offset;   // LDS byte offset of the work-queue words
     uint32_t chunk_cap;            // channels per LDS pass at most (0: each engine's own)
-    const uint32_t *seg_of_block;  // the segment holding item k * SEG_BLOCK, for k <= total items / SEG_BLOCK
+    const uint32_t *seg_of_item;   // the segment of every work item (item_segment_kernel)
     uint32_t wait_limit;           // polls of a gain word before a tile gives up
     uint32_t *gave_up;             // set by the first tile that gives up; later waits end at once
     const uint32_t *abort_word;    // host-written (pinned, coherent), or null
 };
-constexpr uint32_t SEG_BLOCK = 256;
-
-// The segment holding `item`: the cursor's segment or the next one, else from the table of every SEG_BLOCK-th
-// item's segment and a walk of up to four segments forward (segments hold their items in order), then a binary
-// search over the rest if the walk runs long.  Each step's loads are independent of one another and issue
-// together: a workgroup's first item finds its segment in three round trips on a cold cache instead of six.
-__device__ __forceinline__ uint32_t find_segment(const BatchState &S, uint64_t item, uint32_t hint) {
-    const auto segs = gmem(S.segs);
-    auto first_of = [&](uint32_t k) -> uint64_t { return k < S.nsegs ? segs[k].first_item : ~0ull; };
-    {
-        const uint64_t f0 = first_of(hint), f1 = first_of(hint + 1), f2 = first_of(hint + 2);
-        if (f0 <= item && item < f1) return hint;
-        if (f1 <= item && item < f2) return hint + 1;
-    }
-    uint32_t lo = gmem(S.seg_of_block)[item / SEG_BLOCK];  // first_item[lo] <= item
-    uint64_t f[4];
-#pragma unroll
-    for (uint32_t k = 0; k < 4; k++) f[k] = first_of(lo + 1 + k);
-#pragma unroll
-    for (uint32_t k = 0; k < 4; k++)
-        if (item < f[k]) return lo + k;
-    lo += 4;
-    uint32_t hi = S.nsegs;  // first_item[lo] <= item < first_item[hi]
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) / 2;
-        if (segs[mid].first_item <= item) lo = mid; else hi = mid;
-    }
-    return lo;
+// The segment of every work item of a batch, so that the persistent kernel finds an item's segment with one load:
+// one workgroup per segment.
+__global__ __launch_bounds__(256) void item_segment_kernel(const DevSegment *segs, uint32_t nsegs, uint32_t *out) {
+    const uint32_t s = blockIdx.x;
+    if (s >= nsegs) return;
+    const DevSegment g = segs[s];
+    const uint64_t n = (uint64_t)g.epochs * g.tiles;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) out[g.first_item + i] = s;
 }
 
 // Agc.account over one window (gain.zig:19-35, called per window as engine/run.zig:240-246 does): the counts
@@ -409,15 +389,11 @@ __global__ __launch_bounds__(TileGeo<TB>::THREADS, 2) void synth_batch_kernel(Ba
     // running workgroup, so the earliest unfinished item never waits on an item that no running workgroup holds,
     // whatever else shares the device.
     constexpr uint32_t QT = THREADS - 1;
-    uint32_t hint = 0;  // the queue thread's segment cursor
     if (threadIdx.x == QT) {
         const unsigned long long item = S.direct_first ? S.first_item + blockIdx.x : atomicAdd(S.queue, 1ull);
         q[0] = (uint32_t)item;
         q[1] = (uint32_t)(item >> 32);
-        if (item < S.total_items) {
-            hint = find_segment(S, item, hint);
-            q[2] = hint;
-        }
+        if (item < S.total_items) q[2] = gmem(S.seg_of_item)[item];
     }
     for (;;) {
         __syncthreads();
@@ -459,17 +435,18 @@ __global__ __launch_bounds__(TileGeo<TB>::THREADS, 2) void synth_batch_kernel(Ba
         auto before_loop = [&]() {
             if (threadIdx.x == QT) next_item = atomicAdd(S.queue, 1ull);
         };
+        // The next item's segment is loaded once the sums are done and published after the tile: its load returns
+        // while the tile quantises, where a search there held every thread at the next barrier for its round trips.
+        uint32_t next_seg = 0;
         auto after_sums = [&]() {
             if (threadIdx.x == QT) {
                 q[0] = (uint32_t)next_item;
                 q[1] = (uint32_t)(next_item >> 32);
-                if (next_item < S.total_items) {
-                    hint = find_segment(S, next_item, hint);
-                    q[2] = hint;
-                }
+                if (next_item < S.total_items) next_seg = gmem(S.seg_of_item)[next_item];
             }
         };
         const uint32_t clips = synth_tile<F, TB>(A, smem, gain_fn, before_loop, after_sums, S.err);
+        if (threadIdx.x == QT) q[2] = next_seg;
         if (threadIdx.x == 0) {
             // One atomic counts the tile and its clips.  Under the adaptive gain the last tile of the segment gets
             // the segment's total and runs the AGC step; under a constant gain nothing waits for the count, and the
@@ -1369,11 +1346,12 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     auto carve = [&](size_t n) { size_t o = (total + 255) & ~(size_t)255; total = o + n; return Piece{o, n}; };
     Piece p_jobs = carve(sizeof(DevJob) * njobs), p_segs = carve(sizeof(DevSegment) * nsegs),
           p_next = carve(sizeof(uint32_t) * nsegs), p_queue = carve(2 * sizeof(unsigned long long)),
-          p_gain = carve(sizeof(unsigned long long) * nsegs), p_sob = carve(sizeof(uint32_t) * (items / SEG_BLOCK + 1)),
+          p_gain = carve(sizeof(unsigned long long) * nsegs),
           p_agc = carve(sizeof(DevAgc) * njobs), p_count = carve(sizeof(unsigned long long) * nsegs),
           p_err = carve(64);
     const size_t image = total;
     Piece p_adj = carve(sizeof(DevAdjust) * (size_t)adjust_cap * njobs);
+    Piece p_soi = carve(sizeof(uint32_t) * std::max<uint64_t>(items, 1));  // filled on the device
     // The pre-pass serves jobs whose epochs span several tiles; with one tile per epoch a row's constants are used
     // once, and the tile computes them itself.
     std::vector<size_t> prep_at(njobs, SIZE_MAX);
@@ -1406,15 +1384,6 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
         memcpy(host + p_gain.off + k * sizeof w, &w, sizeof w);
     }
     memcpy(host + p_agc.off, agc.data(), p_agc.size);
-    {
-        uint32_t *sob = (uint32_t *)(host + p_sob.off);
-        uint32_t k = 0;
-        for (uint64_t b = 0; b <= items / SEG_BLOCK; b++) {
-            const uint64_t it = b * SEG_BLOCK;
-            while (k + 1 < nsegs && segs[k + 1].first_item <= it) k++;
-            sob[b] = k;
-        }
-    }
     void *base;
     int st = batch_alloc(h, &base, total);
     if (st) { batch_free(h); return st; }
@@ -1443,7 +1412,7 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     S.adjust_cap = adjust_cap;
     S.err = (uint32_t *)(b8 + p_err.off);
     S.gave_up = (uint32_t *)(b8 + p_err.off + 4);
-    S.seg_of_block = (const uint32_t *)(b8 + p_sob.off);
+    S.seg_of_item = (const uint32_t *)(b8 + p_soi.off);
     S.wait_limit = tuning && tuning->wait_limit ? tuning->wait_limit : (1u << 24);
     h->abort_slot = abort_take();
     S.abort_word = h->abort_slot >= 0 ? g_abort_dev + h->abort_slot : nullptr;
@@ -1453,7 +1422,15 @@ static int batch_submit_body(const pharos_synth_job *jobs, uint32_t njobs, const
     h->d_seg_count = S.seg_count;
     h->d_err = S.err;
 
-    // The pre-pass: each job's per-row rotor constants, once for all of a row's tiles.
+    // Each item's segment, then the pre-pass: each job's per-row rotor constants, once for all of a row's tiles.
+    hipLaunchKernelGGL(item_segment_kernel, dim3(nsegs), dim3(256), 0, h->stream, S.segs, nsegs,
+                       (uint32_t *)(b8 + p_soi.off));
+    if (hipGetLastError() != hipSuccess) {
+        set_error("item segment launch failed");
+        (void)hipStreamSynchronize(h->stream);
+        batch_free(h);
+        return ST_INTERNAL;
+    }
     for (uint32_t j = 0; j < njobs; j++) {
         const uint64_t n = (uint64_t)jobs[j].epochs * jobs[j].channels;
         if (n == 0 || prep_at[j] == SIZE_MAX) continue;

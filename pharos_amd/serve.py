@@ -38,6 +38,7 @@ RESERVE = 8 << 30       # HBM left free for the library's own allocations: engin
                         # caches between batches (at most 4 GiB of them); each request's share of a batch's state
                         # is held against the arena while it is admitted (Request.state_bytes)
 PREP_BYTES = 272        # the batched form's row pre-pass per row (RowPrep) of a job with several tiles per epoch
+ITEM_BYTES = 4          # the batched form's segment of each work item (a 4096-sample tile of an epoch)
 MIN_COPY = 64 << 10     # bytes a copy carries at least: the runtime moves a copy this large with a DMA engine, beside
                         # a running batch; a small one (4 KiB measured) goes to a copy kernel, which waits for the
                         # batch to leave the compute units
@@ -84,11 +85,13 @@ class Request:
 
     def state_bytes(self):
         """Device bytes of this request's share of a batch's state: the row pre-pass of one slice (when an epoch
-        spans several tiles) and its job and segment records, with the rounding of the block they live in."""
+        spans several tiles), its work items' segments and its job and segment records, with the rounding of the
+        block they live in."""
         rows = self.slice_epochs * max(1, len(self.scenario.channels))
         prep = rows * PREP_BYTES if self.scenario.samples_per_epoch > 4096 else 0
+        items = self.slice_epochs * -(-self.scenario.samples_per_epoch // 4096) * ITEM_BYTES
         segments = self.slice_epochs // 100 + 2
-        return STATE_SLACK * (prep + 256 + segments * 64)
+        return STATE_SLACK * (prep + items + 256 + segments * 64)
 
     def spec(self):
         """The scenario's ctypes records (built once: slices reuse them)."""

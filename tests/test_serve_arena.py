@@ -82,6 +82,8 @@ def test_request_state_share_covers_the_row_prepass():
         r = serve.Request(sc)
         rows = r.slice_epochs * len(sc.channels)
         prep = rows * serve.PREP_BYTES if sc.samples_per_epoch > 4096 else 0
-        # the share covers the pre-pass with the block's rounding (below 256 MiB a power of two: at most 2x)
-        assert r.state_bytes() >= 2 * prep
-        assert r.state_bytes() <= 2 * prep + 64 * (r.slice_epochs // 100 + 2) * 2 + 1024
+        items = r.slice_epochs * -(-sc.samples_per_epoch // 4096) * serve.ITEM_BYTES
+        # the share covers the pre-pass and the items' segments with the block's rounding (below 256 MiB a power
+        # of two: at most 2x)
+        assert r.state_bytes() >= 2 * (prep + items)
+        assert r.state_bytes() <= 2 * (prep + items) + 64 * (r.slice_epochs // 100 + 2) * 2 + 1024
