@@ -39,19 +39,35 @@ __device__ __forceinline__ double div_by_const(double x, double d, double inv_d)
     return fma(fma(-q0, d, x), inv_d, q0);
 }
 
-__device__ __forceinline__ void sincos64(double phi, const double *__restrict__ turn, double &c, double &s) {
+// sincos64 in three steps, so that several evaluations can issue their table loads together: the turn-table index
+// and the residual's Taylor terms, the table entry, and the rotation of the entry by the residual.
+struct TurnArg {
+    uint32_t i;
+    double cr, sr;
+};
+__device__ __forceinline__ TurnArg turn_reduce(double phi) {
     double t = phi - floor(phi);
     double u = t * 4096.0;
     double fu = floor(u);
-    uint32_t i = (uint32_t)fu & 4095u;
+    TurnArg a;
+    a.i = (uint32_t)fu & 4095u;
     double r = (u - fu) * (2.0 * 3.14159265358979323846 / 4096.0);
     double r2 = r * r;
-    double cr = 1.0 - r2 * (0.5 - r2 * (1.0 / 24.0 - div_by_const(r2, 720.0, 1.0 / 720.0)));
-    double sr = r * (1.0 - r2 * (1.0 / 6.0 - r2 * (1.0 / 120.0 - div_by_const(r2, 5040.0, 1.0 / 5040.0))));
-    const f64x2_t e = *gmem(reinterpret_cast<const f64x2_t *>(turn + 2 * i));
+    a.cr = 1.0 - r2 * (0.5 - r2 * (1.0 / 24.0 - div_by_const(r2, 720.0, 1.0 / 720.0)));
+    a.sr = r * (1.0 - r2 * (1.0 / 6.0 - r2 * (1.0 / 120.0 - div_by_const(r2, 5040.0, 1.0 / 5040.0))));
+    return a;
+}
+__device__ __forceinline__ f64x2_t turn_entry(const double *__restrict__ turn, uint32_t i) {
+    return *gmem(reinterpret_cast<const f64x2_t *>(turn + 2 * i));
+}
+__device__ __forceinline__ void turn_rotate(const TurnArg &a, f64x2_t e, double &c, double &s) {
     const double ec = e.x, es = e.y;
-    c = ec * cr - es * sr;
-    s = es * cr + ec * sr;
+    c = ec * a.cr - es * a.sr;
+    s = es * a.cr + ec * a.sr;
+}
+__device__ __forceinline__ void sincos64(double phi, const double *__restrict__ turn, double &c, double &s) {
+    const TurnArg a = turn_reduce(phi);
+    turn_rotate(a, turn_entry(turn, a.i), c, s);
 }
 
 // chip.zig:167-169
@@ -351,15 +367,12 @@ __device__ __forceinline__ void chan_group(ChanRecT<TB> &R, float2 *vt, uint32_t
     const bool first_lane = j == 0;
     if (first_lane) R.active = 0;
     if (row.table == PHAROS_SYNTH_INACTIVE) return;
-#pragma unroll
-    for (uint32_t b = j; b < TB; b += G) {
-        double c, s;
-        sincos64(row.carrier_phase0 + (double)(n0 + b * RESEED) * row.carrier_step, E.turn, c, s);
-        R.c0_c[b] = c;
-        R.c0_s[b] = s;
-    }
-    if (row.table >= E.ntables) { if (first_lane) atomicOr(err, ERR_TABLE); return; }
-    const auto &T = gmem(E.tables)[row.table];
+    // The code table's descriptor (at an index clamped into the table set, checked below; an engine's table array
+    // holds at least one record) and the block seeds' turn-table entries are loaded together, ahead of the
+    // arithmetic that waits for them: the seed loop has a fixed trip count (j < G), so no branch splits its loads
+    // from one another.
+    static_assert(TB % G == 0, "a channel's threads split its rotor blocks evenly");
+    const auto &T = gmem(E.tables)[min(row.table, max(E.ntables, 1u) - 1u)];
     const uint32_t per = T.per, w_max = TB == TILE_BLOCKS ? T.w_max : T.w_max_wide, ext_len = T.ext_len;
     const uint32_t axis = T.axis, tdm = T.tdm_periods, count = T.count;
     const uint8_t *ext = T.ext;
@@ -371,15 +384,41 @@ __device__ __forceinline__ void chan_group(ChanRecT<TB> &R, float2 *vt, uint32_t
         vre[i] = k0 + i < 8 ? T.vre[k0 + i] : 0.0;
         vim[i] = k0 + i < 8 ? T.vim[k0 + i] : 0.0;
     }
-    double w1c = 0.0, w1s = 0.0;
-    float w_re, w_im;
+    constexpr uint32_t NSEED = TB / G;
+    TurnArg sa[NSEED];
+    f64x2_t se[NSEED];
+#pragma unroll
+    for (uint32_t k = 0; k < NSEED; k++)
+        sa[k] = turn_reduce(row.carrier_phase0 + (double)(n0 + (j + k * G) * RESEED) * row.carrier_step);
+#pragma unroll
+    for (uint32_t k = 0; k < NSEED; k++) se[k] = turn_entry(E.turn, sa[k].i);
+    // w16 in f32 from the row's pre-pass, or, without one, w1 and w16 derived by the tile: either way their loads
+    // go with the seeds'.
+    TurnArg wa[2] = {};
+    f64x2_t we[2] = {};
+    float w_re = 0.0f, w_im = 0.0f;
     if (prep) {
         w_re = gmem(prep)->w_re;
         w_im = gmem(prep)->w_im;
     } else {
+        wa[0] = turn_reduce(row.carrier_step);
+        wa[1] = turn_reduce(16.0 * row.carrier_step);
+        we[0] = turn_entry(E.turn, wa[0].i);
+        we[1] = turn_entry(E.turn, wa[1].i);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < NSEED; k++) {
+        double c, s;
+        turn_rotate(sa[k], se[k], c, s);
+        R.c0_c[j + k * G] = c;
+        R.c0_s[j + k * G] = s;
+    }
+    if (row.table >= E.ntables) { if (first_lane) atomicOr(err, ERR_TABLE); return; }
+    double w1c = 0.0, w1s = 0.0;
+    if (!prep) {
         double wc, ws;
-        sincos64(row.carrier_step, E.turn, w1c, w1s);
-        sincos64(16.0 * row.carrier_step, E.turn, wc, ws);
+        turn_rotate(wa[0], we[0], w1c, w1s);
+        turn_rotate(wa[1], we[1], wc, ws);
         w_re = (float)wc;
         w_im = (float)ws;
     }
